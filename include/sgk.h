@@ -1,0 +1,174 @@
+/*
+ * sgk.h - kernel-level C-ABI of the MI355X per-step MD engine (library libsurfacegrowth.so).
+ *
+ * The reference has no such layer: its wrapper DoComputationsW (SurfaceGrowth.cu:2013-2529)
+ * launches the kernels directly.  This layer exposes each stage of the reference step so a host
+ * (the drop-in wrappers in surfacegrowth.h, tests, benchmarks) can drive it stage by stage or as a
+ * device-resident loop.  Each stage cites the reference kernel whose results it reproduces.
+ *
+ * All functions return SGK_OK (0) or a negative SGK_E* code; sgk_last_error() gives the message.
+ * Host arrays are plain pointers in the reference layouts: positions float4 (x,y,z,E_pot),
+ * velocities and accelerations float3, atom order metal [0,nMolMe) then carbon [nMolMe,nMol).
+ */
+#ifndef SGK_MI355X_H
+#define SGK_MI355X_H
+
+#include "surfacegrowth.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SGK_OK = 0,
+    SGK_E_ARG = -1,        /* invalid argument */
+    SGK_E_HIP = -2,        /* HIP runtime error */
+    SGK_E_NOMEM = -3,      /* device allocation failed */
+    SGK_E_OVERFLOW = -4,   /* a cell or neighbour list exceeded its capacity */
+    SGK_E_UNSUPPORTED = -5,/* input outside what the engine implements */
+    SGK_E_STATE = -6,      /* call made in the wrong order (e.g. no neighbour list yet) */
+    SGK_E_DISPLACEMENT = -7/* an atom moved further than the neighbour-list skin allows */
+};
+
+/* Device error flags accumulated by the kernels (bitmask in sgk_record.errors). */
+enum {
+    SGK_FLAG_CELL_OVERFLOW = 1,   /* more than cell_capacity atoms in one cell */
+    SGK_FLAG_NL_OVERFLOW = 2,     /* more than iNebrMax neighbours (reference: silent corruption) */
+    SGK_FLAG_LJ_OVERFLOW = 4,     /* carbon: more metal neighbours than lj_capacity */
+    SGK_FLAG_CLOSE_OVERFLOW = 8,  /* carbon: more close neighbours than close_capacity */
+    SGK_FLAG_UNBINNED = 16,       /* an atom's cell index fell outside the grid (reference drops it) */
+    SGK_FLAG_RDF_EDGE = 32,       /* a pair rounded into bin sizeHistRdf (reference writes past the end) */
+    SGK_FLAG_DISPLACEMENT = 64    /* skin bound for the carbon close lists exceeded */
+};
+
+typedef struct sgk_ctx sgk_ctx;
+
+typedef struct sgk_config {
+    int precision;        /* SG_PREC_FP32: FP32 state and sums, as the reference.
+                             SG_PREC_MIXED: FP64 positions/velocities and FP64 sums, FP32 pair math */
+    int nl_capacity;      /* neighbours per atom; 0 -> SimParams.iNebrMax (reference 128) */
+    int cell_capacity;    /* atoms per cell; 0 -> 64 (reference BLOCK_SIZE) */
+    int lj_capacity;      /* metal neighbours kept per carbon atom; 0 -> 64 */
+    int close_capacity;   /* close neighbours kept per carbon atom; 0 -> 16 */
+    float close_margin;   /* close radius = rCutC + 2*(0.5*rNebrShell + margin); 0 -> 0.2 */
+    int metal_lanes;      /* lanes per metal atom in the rho/force kernels (1..16); 0 -> auto */
+    int block_size;       /* threads per block for per-atom kernels; 0 -> 256 */
+    int keep_full_nl;     /* also store the neighbour list in the reference layout (tests) */
+    int rebuild_every;    /* 0: adaptive rule dispHi > rNebrShell/2 only; k>0: also every k steps */
+    int records_capacity; /* step records buffered on the device; 0 -> 1024 */
+    int use_graphs;       /* capture the device loop in hipGraphs */
+    int metal_thermostat_all; /* thermostat metal after stepEquil too (reference: only while
+                                 stepEquil < step < stepEquil + stepCool) */
+    int pair_fp64;        /* mixed form only: EAM and LJ pair arithmetic in FP64 (default) or FP32 */
+} sgk_config;
+
+/* One record per step, written on the device by the reduction stage.  Sums are over atoms with
+ * the reference's semantics (SurfaceGrowth.cu:1101-1891); see sgk_eval_props. */
+typedef struct sgk_record {
+    int step;             /* stepCount after the step */
+    int rebuilt;          /* the neighbour list was rebuilt in this step */
+    int nebrNow;          /* rebuild requested for the next step */
+    int errors;           /* SGK_FLAG_* raised so far */
+    double kinEnergy;     /* 0.5*vvSum/nMol */
+    double potEnergy;     /* uSum/nMol */
+    double totEnergy;
+    double vSum[3];       /* sum v, metal x component times massMe (ComputeVSumK) */
+    double vvSum;         /* sum m|v - vcm_x|^2 (ComputeVvSumK) */
+    double vvMax;         /* max |v|^2 (ComputeVvMaxK) */
+    double uSum;          /* sum E_pot (ComputePotEnergyK) */
+    double comSum[3];     /* sum r over metal (ComputeCenterOfMassK) */
+    double cmVelSum[3];   /* sum v over metal (ComputeCmVelK) */
+    double frict;         /* sum friction x over metal (ComputeNetForceK) */
+    double bboxMin[3];    /* min of metal r including the reference's zero padding */
+    double bboxMax[3];
+    double dispHi;        /* after this step */
+    double shear;         /* current shear increment state (regime 2) */
+    double shearCount;    /* atoms sheared in this step */
+} sgk_record;
+
+void sgk_default_config(sgk_config *cfg);
+
+/* Create an engine for the parameters in *p (copied; the struct is the reference SimParams). */
+int sgk_create(const SimParams *p, const sgk_config *cfg, sgk_ctx **out);
+int sgk_destroy(sgk_ctx *ctx);
+/* Message for the last failing call on ctx (ctx may be NULL for creation errors). */
+const char *sgk_last_error(const sgk_ctx *ctx);
+/* Number of HIP devices (0 when no runtime/GPU); does not initialise a context. */
+int sgk_device_count(void);
+
+/* State transfer (host reference layout <-> device).  In mixed precision the float inputs are
+ * widened; the _f64 variants move full FP64 state (r: 4 doubles x,y,z,E per atom; v,a: 3). */
+int sgk_upload(sgk_ctx *ctx, const sg_float4 *r, const sg_float3 *v, const sg_float3 *a);
+int sgk_download(sgk_ctx *ctx, sg_float4 *r, sg_float3 *v, sg_float3 *a);
+int sgk_upload_f64(sgk_ctx *ctx, const double *r, const double *v, const double *a);
+int sgk_download_f64(sgk_ctx *ctx, double *r, double *v, double *a);
+/* Step counter, rebuild flag and displacement accumulator of the run state. */
+int sgk_set_run_state(sgk_ctx *ctx, int stepCount, int nebrNow, double dispHi);
+
+/* --- stages ---------------------------------------------------------------------------------- */
+/* Cell binning (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609): cell index per atom with the
+ * reference FP32 formula; cells listed in ascending atom order. */
+int sgk_bin(sgk_ctx *ctx);
+/* cellOf[nMol] (-1 = outside the grid), cellCount[ncell], cellSlots[ncell*cell_capacity]. */
+int sgk_get_cells(sgk_ctx *ctx, int *cellOf, int *cellCount, int *cellSlots);
+/* Neighbour list (BuildNebrListK, SurfaceGrowth.cu:612-711) from the current cells. */
+int sgk_build_nebr(sgk_ctx *ctx);
+/* Reference layout NN[nMol], NBL[iNebrMax*nMol] column-major, -1 filled; needs keep_full_nl. */
+int sgk_get_nebr(sgk_ctx *ctx, int *NN, int *NBL);
+/* Per-carbon lists the force kernel uses: metal neighbours and close neighbours, each entry
+ * (atom index, position in the reference list).  Arrays sized nC*capacity, row per carbon. */
+int sgk_get_carbon_lists(sgk_ctx *ctx, int *nLj, int *ljIdx, int *ljPos, int *nClose,
+                         int *closeIdx, int *closePos);
+/* Electron density and embedding terms (EamComputeRhoK, SurfaceGrowth.cu:716-773). */
+int sgk_compute_rho(sgk_ctx *ctx);
+/* Forces, per-atom energy, friction (ComputeForcesK, SurfaceGrowth.cu:776-1091) without
+ * integration. */
+int sgk_compute_forces(sgk_ctx *ctx);
+/* rho[nMolMe]; acc[3*nMol]; energy[nMol] (the .w of r); fric[nMolMe].  NULL pointers skipped. */
+int sgk_get_rho(sgk_ctx *ctx, double *rho);
+int sgk_get_forces(sgk_ctx *ctx, double *acc, double *energy, double *fric);
+/* LeapfrogStepK (SurfaceGrowth.cu:397-434) part 1 or 2; part 1 is followed by
+ * ApplyBoundaryCondK (SurfaceGrowth.cu:436-453). */
+int sgk_leapfrog(sgk_ctx *ctx, int part);
+/* ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485) with the given vvSum. */
+int sgk_thermostat(sgk_ctx *ctx, int stepCount, double vvSum);
+/* Reductions of the current state with the reference semantics (SurfaceGrowth.cu:1101-1891);
+ * cmVelX for the regime-2 kinetic energy is this state's own metal centre-of-mass velocity. */
+int sgk_eval_props(sgk_ctx *ctx, sgk_record *rec);
+/* RDF histogram (EvalRdfK, SurfaceGrowth.cu:1895-1999) over the cells of the last binning. */
+int sgk_rdf_clear(sgk_ctx *ctx);
+int sgk_rdf_accumulate(sgk_ctx *ctx);
+int sgk_rdf_get(sgk_ctx *ctx, unsigned int *hist);
+
+/* --- device-resident loop --------------------------------------------------------------------- */
+/* Advance nsteps steps in the order of DoComputationsW (SurfaceGrowth.cu:2141-2489) without host
+ * synchronisation between steps: leapfrog 1 + boundary, rebuild when flagged on the device,
+ * density, forces fused with leapfrog 2 and the reductions, thermostat on its steps, shear,
+ * RDF accumulation every rdfEvery steps (0: none).  records[nsteps] receives one sgk_record per
+ * step (may be NULL).  nsteps must not exceed records_capacity. */
+int sgk_run(sgk_ctx *ctx, int nsteps, int rdfEvery, sgk_record *records);
+/* Milliseconds of device time spent in the last sgk_run (HIP events on the engine stream). */
+double sgk_last_run_ms(sgk_ctx *ctx);
+
+/* --- introspection for benchmarks ------------------------------------------------------------ */
+typedef struct sgk_kernel_stat {
+    char name[48];
+    int launches;
+    double ms;            /* accumulated device time when profiling is on */
+} sgk_kernel_stat;
+/* Turn per-kernel event timing on/off (adds a sync per kernel; for measurement only). */
+int sgk_set_profiling(sgk_ctx *ctx, int on);
+/* Copy up to max kernel stats; returns the number of kernels. */
+int sgk_kernel_stats(sgk_ctx *ctx, sgk_kernel_stat *stats, int max);
+/* Bytes of device memory owned by ctx. */
+long long sgk_device_bytes(sgk_ctx *ctx);
+/* sizeof of sgk_config, sgk_record, sgk_kernel_stat (for bindings); returns the number written. */
+int sgk_struct_sizes(int *out, int max);
+/* Lanes per metal atom chosen at creation. */
+int sgk_metal_lanes(const sgk_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGK_MI355X_H */

@@ -1,0 +1,808 @@
+// sgk_api.hip - the sgk_* C-ABI: engine context, state transfer, stages and the device-resident
+// step loop (hipGraph-captured chunks of steps).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sgk_device.h"
+#include "sgk_internal.h"
+
+using sgk::DevParams;
+using sgk::DevState;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(sgk_ctx *c, int code, const std::string &msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int hip_fail(sgk_ctx *c, hipError_t e, const char *what) {
+    return fail(c, SGK_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define SGK_HIP(c, expr)                                   \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+    } while (0)
+
+template <class T>
+int dalloc(sgk_ctx *c, T **p, size_t count, bool zero = false) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return fail(c, SGK_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed");
+    c->bytes += (long long)bytes;
+    if (zero) {
+        e = hipMemsetAsync(*p, 0, bytes, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "hipMemsetAsync");
+    }
+    return SGK_OK;
+}
+
+// Fatal device flags: capacities exceeded or the carbon close lists no longer exact.
+const int kFatal = SGK_FLAG_CELL_OVERFLOW | SGK_FLAG_NL_OVERFLOW | SGK_FLAG_LJ_OVERFLOW | SGK_FLAG_CLOSE_OVERFLOW |
+                   SGK_FLAG_DISPLACEMENT;
+
+std::string flag_text(int f) {
+    std::string s;
+    if (f & SGK_FLAG_CELL_OVERFLOW) s += " cell capacity exceeded;";
+    if (f & SGK_FLAG_NL_OVERFLOW) s += " neighbour-list capacity (iNebrMax) exceeded;";
+    if (f & SGK_FLAG_LJ_OVERFLOW) s += " carbon metal-neighbour capacity exceeded;";
+    if (f & SGK_FLAG_CLOSE_OVERFLOW) s += " carbon close-neighbour capacity exceeded;";
+    if (f & SGK_FLAG_DISPLACEMENT) s += " displacement since the last rebuild exceeds the skin bound;";
+    return s;
+}
+
+int check_flags(sgk_ctx *c) {
+    hipError_t e = hipMemcpyAsync(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpyAsync(state)");
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+    const int f = c->hostState->errors & kFatal;
+    if (f) return fail(c, (f & SGK_FLAG_DISPLACEMENT) ? SGK_E_DISPLACEMENT : SGK_E_OVERFLOW, "device:" + flag_text(f));
+    return SGK_OK;
+}
+
+// Per-kernel event timing (profiling mode only; never during graph capture).
+struct PendingEvent {
+    std::string name;
+    hipEvent_t a, b;
+};
+thread_local std::vector<hipEvent_t> g_event_pool;
+thread_local std::vector<PendingEvent> g_pending;
+thread_local hipEvent_t g_open = nullptr;
+thread_local bool g_capturing = false;
+
+hipEvent_t take_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void flush_profile(sgk_ctx *c) {
+    if (g_pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &p : g_pending) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        auto &s = c->stats[p.name];
+        s.launches += 1;
+        s.ms += ms;
+        g_event_pool.push_back(p.a);
+        g_event_pool.push_back(p.b);
+    }
+    g_pending.clear();
+}
+
+}  // namespace
+
+namespace sgk {
+
+void profile_begin(sgk_ctx *c) {
+    if (!c->profiling || g_capturing) return;
+    g_open = take_event();
+    (void)hipEventRecord(g_open, c->stream);
+}
+
+void profile_end(sgk_ctx *c, const char *name) {
+    if (!c->profiling || g_capturing || !g_open) return;
+    hipEvent_t b = take_event();
+    (void)hipEventRecord(b, c->stream);
+    g_pending.push_back({name, g_open, b});
+    g_open = nullptr;
+}
+
+}  // namespace sgk
+
+static void fill_dev_params(sgk_ctx *c) {
+    const SimParams &s = c->sp;
+    DevParams &p = c->dp;
+    std::memset(&p, 0, sizeof(p));
+    p.nMol = s.nMol;
+    p.nMolMe = s.nMolMe;
+    p.nC = s.nMol - s.nMolMe;
+    p.iRegime = s.iRegime;
+    p.Lx = s.region.x;
+    p.Ly = s.region.y;
+    p.Lz = s.region.z;
+    p.cx = s.cells.x;
+    p.cy = s.cells.y;
+    p.cz = s.cells.z;
+    p.ncell = s.cells.x * s.cells.y * s.cells.z;
+    p.iwx = s.invWidth.x;
+    p.iwy = s.invWidth.y;
+    p.iwz = s.invWidth.z;
+    p.rrNebr = s.rrNebr;
+    p.rrCutEam = s.rrCutEam;
+    p.rrCutLJ = s.rrCutLJ;
+    p.rrCutC = s.rrCutC;
+    const float margin = c->cfg.close_margin;
+    const float closeR = s.rCutC + 2.f * (0.5f * s.rNebrShell + margin);
+    p.closeR2 = closeR * closeR;
+    p.dispBound = 0.5f * s.rNebrShell + margin;
+    p.re = s.re;
+    p.fe = s.fe;
+    p.rhoe = s.rhoe;
+    p.alpha = s.alpha;
+    p.beta = s.beta;
+    p.A = s.A;
+    p.B = s.B;
+    p.kappa = s.kappa;
+    p.lambda = s.lambda;
+    for (int k = 0; k < 4; ++k) {
+        p.Fn[k] = s.Fn[k];
+        p.F[k] = s.F[k];
+    }
+    p.eta = s.eta;
+    p.Fe = s.Fe;
+    p.massMe = s.massMe;
+    p.rei = s.rei;
+    p.rhoei = s.rhoei;
+    p.mu_r = s.mu_r;
+    p.theta_0 = s.theta_0;
+    p.mu_theta = s.mu_theta;
+    p.mu_p = s.mu_p;
+    p.z_0 = s.z_0;
+    p.epsLJ = s.epsLJ;
+    p.sigma2 = s.sigmaLJ * s.sigmaLJ;
+    p.inv_sigma2 = 1.f / p.sigma2;
+    p.deltaT = s.deltaT;
+    p.temperature = s.temperature;
+    p.kB = s.kB;
+    p.gamma = s.gammaBerendsen;
+    p.rNebrShell = s.rNebrShell;
+    p.stepThermostat = s.stepThermostat;
+    p.stepEquil = s.stepEquil;
+    p.stepCool = s.stepCool;
+    // frozen perimeter bounds (SurfaceGrowth.cu:415-421); cos(M_PI/6.f) is evaluated in double
+    const float cos30 = (float)std::cos(M_PI / 6.0);
+    p.xLeft = 0.5f * (cos30 - s.region.x);
+    p.xRight = 0.5f * (s.region.x - cos30);
+    p.yBottom = 0.5f * (1.f - s.region.y);
+    p.yTop = 0.5f * (s.region.y - 1.f);
+    p.rangeRdf2 = s.rangeRdf * s.rangeRdf;
+    p.intervalRdf = s.intervalRdf;
+    p.sizeHistRdf = s.sizeHistRdf;
+    p.deltaF = s.deltaF;
+    p.cellCap = c->cfg.cell_capacity;
+    p.nlCap = c->cfg.nl_capacity;
+    p.ljCap = c->cfg.lj_capacity;
+    p.clCap = c->cfg.close_capacity;
+    p.keepFull = c->cfg.keep_full_nl;
+    p.rebuildEvery = c->cfg.rebuild_every;
+}
+
+extern "C" {
+
+void sgk_default_config(sgk_config *cfg) {
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->precision = SG_PREC_FP32;
+    cfg->cell_capacity = 64;
+    cfg->lj_capacity = 64;
+    cfg->close_capacity = 16;
+    cfg->close_margin = 0.2f;
+    cfg->block_size = 256;
+    cfg->records_capacity = 1024;
+    cfg->use_graphs = 1;
+    cfg->pair_fp64 = 1;  // mixed form: FP64 pair arithmetic (measured cost nil, error < 1e-5)
+}
+
+int sgk_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *sgk_last_error(const sgk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
+    if (!sp || !out) return fail(nullptr, SGK_E_ARG, "null argument");
+    *out = nullptr;
+    sgk_config cfg;
+    if (cfgIn)
+        cfg = *cfgIn;
+    else
+        sgk_default_config(&cfg);
+    sgk_config def;
+    sgk_default_config(&def);
+    if (cfg.nl_capacity <= 0) cfg.nl_capacity = sp->iNebrMax > 0 ? sp->iNebrMax : 128;
+    if (cfg.cell_capacity <= 0) cfg.cell_capacity = def.cell_capacity;
+    if (cfg.lj_capacity <= 0) cfg.lj_capacity = def.lj_capacity;
+    if (cfg.close_capacity <= 0) cfg.close_capacity = def.close_capacity;
+    if (cfg.close_margin <= 0.f) cfg.close_margin = def.close_margin;
+    if (cfg.block_size <= 0) cfg.block_size = def.block_size;
+    if (cfg.records_capacity <= 0) cfg.records_capacity = def.records_capacity;
+    if (sp->nMol <= 0 || sp->nMolMe < 0 || sp->nMolMe > sp->nMol)
+        return fail(nullptr, SGK_E_ARG, "invalid atom counts");
+    if (sp->nMol >= (1 << 24)) return fail(nullptr, SGK_E_UNSUPPORTED, "more than 2^24 atoms");
+    if (cfg.nl_capacity > 255) return fail(nullptr, SGK_E_UNSUPPORTED, "neighbour capacity above 255");
+    if (cfg.cell_capacity > 64) return fail(nullptr, SGK_E_UNSUPPORTED, "cell capacity above 64 (one wave per cell)");
+    if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return fail(nullptr, SGK_E_ARG, "empty cell grid");
+    if (sp->iRegime < 0 || sp->iRegime > 2) return fail(nullptr, SGK_E_ARG, "regime must be 0, 1 or 2");
+    if (sp->sizeHistRdf <= 0 || sp->sizeHistRdf > 8192) return fail(nullptr, SGK_E_ARG, "RDF bins must be 1..8192");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, SGK_E_HIP, "no HIP device");
+
+    sgk_ctx *c = new sgk_ctx();
+    c->sp = *sp;
+    c->cfg = cfg;
+    c->mixed = cfg.precision == SG_PREC_MIXED;
+    (void)hipGetDevice(&c->device);
+    int rc;
+#define SGK_TRY(x)                   \
+    do {                             \
+        if ((rc = (x)) != SGK_OK) {  \
+            g_create_error = c->err; \
+            sgk_destroy(c);          \
+            return rc;               \
+        }                            \
+    } while (0)
+    {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+            delete c;
+            return SGK_E_HIP;
+        }
+    }
+    fill_dev_params(c);
+    const DevParams &p = c->dp;
+    // lanes per metal atom: enough waves to cover the machine for small clusters
+    int G = cfg.metal_lanes;
+    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe < 32768 ? 16 : 8;
+    c->metalLanes = G;
+    c->block = 256;
+    c->nbM = (int)(((long long)p.nMolMe * G + 255) / 256);
+    c->nbC = (p.nC + 255) / 256;
+    c->nbProps = (p.nMol + 255) / 256;
+    const size_t R = c->mixed ? 8 : 4;
+    const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
+    SGK_TRY(dalloc(c, (char **)&c->pos, nMol * (c->mixed ? 32 : 16), true));
+    SGK_TRY(dalloc(c, (char **)&c->vel, 3 * nMol * R, true));
+    SGK_TRY(dalloc(c, &c->acc, 3 * nMol, true));
+    SGK_TRY(dalloc(c, &c->rho, nMe, true));
+    SGK_TRY(dalloc(c, &c->embF, nMe, true));
+    SGK_TRY(dalloc(c, &c->embDF, nMe, true));
+    SGK_TRY(dalloc(c, &c->fric, nMe, true));
+    SGK_TRY(dalloc(c, &c->cellOf, nMol, true));
+    for (int b = 0; b < 2; ++b) {
+        SGK_TRY(dalloc(c, &c->cnt[b], (size_t)p.ncell, true));
+        SGK_TRY(dalloc(c, &c->slots[b], (size_t)p.ncell * p.cellCap, false));
+        SGK_TRY(dalloc(c, &c->nonempty[b], (size_t)p.ncell, false));
+    }
+    SGK_TRY(dalloc(c, &c->nlM, nMe * (size_t)p.nlCap, false));
+    SGK_TRY(dalloc(c, &c->nnM, nMe, true));
+    SGK_TRY(dalloc(c, &c->lj, nC * (size_t)p.ljCap, false));
+    SGK_TRY(dalloc(c, &c->nlj, nC, true));
+    SGK_TRY(dalloc(c, &c->cl, nC * (size_t)p.clCap, false));
+    SGK_TRY(dalloc(c, &c->ncl, nC, true));
+    if (p.keepFull) {
+        SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
+        SGK_TRY(dalloc(c, &c->fullNBL, nMol * (size_t)p.nlCap, false));
+    }
+    const size_t nParts = std::max<size_t>((size_t)(c->nbM + c->nbC), (size_t)c->nbProps);
+    SGK_TRY(dalloc(c, (char **)&c->partials, nParts * sgk::partial_bytes(c), true));
+    SGK_TRY(dalloc(c, &c->state, 1, true));
+    SGK_TRY(dalloc(c, &c->records, (size_t)cfg.records_capacity, true));
+    SGK_TRY(dalloc(c, &c->hist, (size_t)p.sizeHistRdf, true));
+    if (hipHostMalloc((void **)&c->hostState, sizeof(DevState), 0) != hipSuccess) {
+        g_create_error = "hipHostMalloc failed";
+        sgk_destroy(c);
+        return SGK_E_NOMEM;
+    }
+    std::memset(c->hostState, 0, sizeof(DevState));
+    c->hostState->nebrNow = 1;
+    c->hostState->step = sp->stepCount;
+    if (hipMemcpyAsync(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        g_create_error = "state initialisation failed";
+        sgk_destroy(c);
+        return SGK_E_HIP;
+    }
+    c->hostStep = sp->stepCount;
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+#undef SGK_TRY
+    *out = c;
+    return SGK_OK;
+}
+
+int sgk_destroy(sgk_ctx *c) {
+    if (!c) return SGK_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->graphs)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf, c->cnt[0], c->cnt[1],
+                    c->slots[0], c->slots[1], c->nonempty[0], c->nonempty[1], c->nlM, c->nnM, c->lj, c->nlj, c->cl,
+                    c->ncl, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c->hostState) (void)hipHostFree(c->hostState);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SGK_OK;
+}
+
+// ---- state transfer --------------------------------------------------------------------------------
+int sgk_upload(sgk_ctx *c, const sg_float4 *r, const sg_float3 *v, const sg_float3 *a) {
+    if (!c) return SGK_E_ARG;
+    const size_t n = (size_t)c->dp.nMol;
+    if (r) {
+        if (c->mixed) {
+            std::vector<double> t(4 * n);
+            for (size_t i = 0; i < n; ++i) {
+                t[4 * i] = r[i].x;
+                t[4 * i + 1] = r[i].y;
+                t[4 * i + 2] = r[i].z;
+                t[4 * i + 3] = r[i].w;
+            }
+            SGK_HIP(c, hipMemcpy(c->pos, t.data(), 32 * n, hipMemcpyHostToDevice));
+        } else {
+            SGK_HIP(c, hipMemcpy(c->pos, r, 16 * n, hipMemcpyHostToDevice));
+        }
+        c->haveList = false;
+    }
+    if (v) {
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            for (size_t i = 0; i < n; ++i) {
+                t[i] = v[i].x;
+                t[n + i] = v[i].y;
+                t[2 * n + i] = v[i].z;
+            }
+            SGK_HIP(c, hipMemcpy(c->vel, t.data(), 8 * 3 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> t(3 * n);
+            for (size_t i = 0; i < n; ++i) {
+                t[i] = v[i].x;
+                t[n + i] = v[i].y;
+                t[2 * n + i] = v[i].z;
+            }
+            SGK_HIP(c, hipMemcpy(c->vel, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+        }
+    }
+    if (a) {
+        std::vector<float> t(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            t[i] = a[i].x;
+            t[n + i] = a[i].y;
+            t[2 * n + i] = a[i].z;
+        }
+        SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+    }
+    return SGK_OK;
+}
+
+int sgk_upload_f64(sgk_ctx *c, const double *r, const double *v, const double *a) {
+    if (!c) return SGK_E_ARG;
+    const size_t n = (size_t)c->dp.nMol;
+    if (r) {
+        if (c->mixed) {
+            SGK_HIP(c, hipMemcpy(c->pos, r, 32 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> t(4 * n);
+            for (size_t i = 0; i < 4 * n; ++i) t[i] = (float)r[i];
+            SGK_HIP(c, hipMemcpy(c->pos, t.data(), 16 * n, hipMemcpyHostToDevice));
+        }
+        c->haveList = false;
+    }
+    if (v) {
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) t[d * n + i] = v[3 * i + d];
+            SGK_HIP(c, hipMemcpy(c->vel, t.data(), 8 * 3 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> t(3 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) t[d * n + i] = (float)v[3 * i + d];
+            SGK_HIP(c, hipMemcpy(c->vel, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+        }
+    }
+    if (a) {
+        std::vector<float> t(3 * n);
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) t[d * n + i] = (float)a[3 * i + d];
+        SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+    }
+    return SGK_OK;
+}
+
+int sgk_download_f64(sgk_ctx *c, double *r, double *v, double *a) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->dp.nMol;
+    if (r) {
+        if (c->mixed) {
+            SGK_HIP(c, hipMemcpy(r, c->pos, 32 * n, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(4 * n);
+            SGK_HIP(c, hipMemcpy(t.data(), c->pos, 16 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < 4 * n; ++i) r[i] = t[i];
+        }
+    }
+    if (v) {
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            SGK_HIP(c, hipMemcpy(t.data(), c->vel, 8 * 3 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) v[3 * i + d] = t[d * n + i];
+        } else {
+            std::vector<float> t(3 * n);
+            SGK_HIP(c, hipMemcpy(t.data(), c->vel, 4 * 3 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) v[3 * i + d] = t[d * n + i];
+        }
+    }
+    if (a) {
+        std::vector<float> t(3 * n);
+        SGK_HIP(c, hipMemcpy(t.data(), c->acc, 4 * 3 * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) a[3 * i + d] = t[d * n + i];
+    }
+    return SGK_OK;
+}
+
+int sgk_download(sgk_ctx *c, sg_float4 *r, sg_float3 *v, sg_float3 *a) {
+    if (!c) return SGK_E_ARG;
+    const size_t n = (size_t)c->dp.nMol;
+    std::vector<double> R(r ? 4 * n : 0), V(v ? 3 * n : 0), A(a ? 3 * n : 0);
+    int rc = sgk_download_f64(c, r ? R.data() : nullptr, v ? V.data() : nullptr, a ? A.data() : nullptr);
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        if (r) r[i] = {(float)R[4 * i], (float)R[4 * i + 1], (float)R[4 * i + 2], (float)R[4 * i + 3]};
+        if (v) v[i] = {(float)V[3 * i], (float)V[3 * i + 1], (float)V[3 * i + 2]};
+        if (a) a[i] = {(float)A[3 * i], (float)A[3 * i + 1], (float)A[3 * i + 2]};
+    }
+    return SGK_OK;
+}
+
+int sgk_set_run_state(sgk_ctx *c, int stepCount, int nebrNow, double dispHi) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
+    c->hostState->step = stepCount;
+    c->hostState->nebrNow = nebrNow;
+    c->hostState->dispHi = dispHi;
+    SGK_HIP(c, hipMemcpy(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice));
+    c->hostStep = stepCount;
+    return SGK_OK;
+}
+
+// ---- stages ------------------------------------------------------------------------------------------
+int sgk_bin(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    sgk::launch_bin(c, true);
+    sgk::launch_sortcells(c, true);
+    sgk::launch_commit_rebuild(c);
+    SGK_HIP(c, hipGetLastError());
+    return check_flags(c);
+}
+
+int sgk_get_cells(sgk_ctx *c, int *cellOf, int *cellCount, int *cellSlots) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
+    const int b = c->hostState->cellbuf;
+    if (cellOf) SGK_HIP(c, hipMemcpy(cellOf, c->cellOf, 4 * (size_t)c->dp.nMol, hipMemcpyDeviceToHost));
+    if (cellCount) SGK_HIP(c, hipMemcpy(cellCount, c->cnt[b], 4 * (size_t)c->dp.ncell, hipMemcpyDeviceToHost));
+    if (cellSlots) {
+        std::vector<int> cnt(c->dp.ncell);
+        SGK_HIP(c, hipMemcpy(cnt.data(), c->cnt[b], 4 * (size_t)c->dp.ncell, hipMemcpyDeviceToHost));
+        SGK_HIP(c, hipMemcpy(cellSlots, c->slots[b], 4 * (size_t)c->dp.ncell * c->dp.cellCap, hipMemcpyDeviceToHost));
+        for (int q = 0; q < c->dp.ncell; ++q)  // slots past the count are undefined: fill -1
+            for (int s = std::min(cnt[q], c->dp.cellCap); s < c->dp.cellCap; ++s) cellSlots[(size_t)q * c->dp.cellCap + s] = -1;
+    }
+    return SGK_OK;
+}
+
+int sgk_build_nebr(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    sgk::launch_nl(c, true);
+    SGK_HIP(c, hipGetLastError());
+    int rc = check_flags(c);
+    if (rc == SGK_OK) c->haveList = true;
+    return rc;
+}
+
+int sgk_get_nebr(sgk_ctx *c, int *NN, int *NBL) {
+    if (!c) return SGK_E_ARG;
+    if (!c->dp.keepFull) return fail(c, SGK_E_STATE, "keep_full_nl was not set at creation");
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->dp.nMol;
+    std::vector<int> nn(n);
+    SGK_HIP(c, hipMemcpy(nn.data(), c->fullNN, 4 * n, hipMemcpyDeviceToHost));
+    if (NN) std::memcpy(NN, nn.data(), 4 * n);
+    if (NBL) {
+        SGK_HIP(c, hipMemcpy(NBL, c->fullNBL, 4 * n * c->dp.nlCap, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            for (int j = nn[i]; j < c->dp.nlCap; ++j) NBL[(size_t)j * n + i] = -1;
+    }
+    return SGK_OK;
+}
+
+int sgk_get_carbon_lists(sgk_ctx *c, int *nLj, int *ljIdx, int *ljPos, int *nClose, int *closeIdx, int *closePos) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t nC = (size_t)c->dp.nC;
+    if (nC == 0) return SGK_OK;
+    std::vector<int> nl(nC), nc(nC);
+    std::vector<unsigned> lj(nC * c->dp.ljCap), cl(nC * c->dp.clCap);
+    SGK_HIP(c, hipMemcpy(nl.data(), c->nlj, 4 * nC, hipMemcpyDeviceToHost));
+    SGK_HIP(c, hipMemcpy(nc.data(), c->ncl, 4 * nC, hipMemcpyDeviceToHost));
+    SGK_HIP(c, hipMemcpy(lj.data(), c->lj, 4 * lj.size(), hipMemcpyDeviceToHost));
+    SGK_HIP(c, hipMemcpy(cl.data(), c->cl, 4 * cl.size(), hipMemcpyDeviceToHost));
+    for (size_t ic = 0; ic < nC; ++ic) {
+        if (nLj) nLj[ic] = nl[ic];
+        if (nClose) nClose[ic] = nc[ic];
+        for (int q = 0; q < c->dp.ljCap; ++q) {
+            const unsigned e = q < nl[ic] ? lj[(size_t)q * nC + ic] : 0xffffffffu;
+            if (ljIdx) ljIdx[ic * c->dp.ljCap + q] = q < nl[ic] ? (int)(e >> 8) : -1;
+            if (ljPos) ljPos[ic * c->dp.ljCap + q] = q < nl[ic] ? (int)(e & 255u) : -1;
+        }
+        for (int q = 0; q < c->dp.clCap; ++q) {
+            const unsigned e = q < nc[ic] ? cl[(size_t)q * nC + ic] : 0xffffffffu;
+            if (closeIdx) closeIdx[ic * c->dp.clCap + q] = q < nc[ic] ? (int)(e >> 8) : -1;
+            if (closePos) closePos[ic * c->dp.clCap + q] = q < nc[ic] ? (int)(e & 255u) : -1;
+        }
+    }
+    return SGK_OK;
+}
+
+int sgk_compute_rho(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    if (!c->haveList) return fail(c, SGK_E_STATE, "no neighbour list: call sgk_bin and sgk_build_nebr first");
+    sgk::launch_rho(c);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_compute_forces(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    if (!c->haveList) return fail(c, SGK_E_STATE, "no neighbour list: call sgk_bin and sgk_build_nebr first");
+    sgk::launch_rho(c);
+    sgk::launch_force(c, false, false);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_get_rho(sgk_ctx *c, double *rho) {
+    if (!c || !rho) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> t(std::max(c->dp.nMolMe, 1));
+    SGK_HIP(c, hipMemcpy(t.data(), c->rho, 4 * (size_t)c->dp.nMolMe, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->dp.nMolMe; ++i) rho[i] = t[i];
+    return SGK_OK;
+}
+
+int sgk_get_forces(sgk_ctx *c, double *acc, double *energy, double *fric) {
+    if (!c) return SGK_E_ARG;
+    const size_t n = (size_t)c->dp.nMol;
+    if (acc) {
+        int rc = sgk_download_f64(c, nullptr, nullptr, acc);
+        if (rc) return rc;
+    }
+    if (energy) {
+        std::vector<double> r(4 * n);
+        int rc = sgk_download_f64(c, r.data(), nullptr, nullptr);
+        if (rc) return rc;
+        for (size_t i = 0; i < n; ++i) energy[i] = r[4 * i + 3];
+    }
+    if (fric && c->dp.nMolMe) {
+        std::vector<float> t(c->dp.nMolMe);
+        SGK_HIP(c, hipMemcpy(t.data(), c->fric, 4 * (size_t)c->dp.nMolMe, hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->dp.nMolMe; ++i) fric[i] = t[i];
+    }
+    return SGK_OK;
+}
+
+int sgk_leapfrog(sgk_ctx *c, int part) {
+    if (!c) return SGK_E_ARG;
+    if (part == 1)
+        sgk::launch_lf1(c, false);
+    else
+        sgk::launch_lf2(c);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_thermostat(sgk_ctx *c, int stepCount, double vvSum) {
+    if (!c) return SGK_E_ARG;
+    sgk::launch_thermostat(c, stepCount, vvSum, 1);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_eval_props(sgk_ctx *c, sgk_record *rec) {
+    if (!c || !rec) return SGK_E_ARG;
+    sgk::launch_props(c);
+    sgk::launch_finalize(c, 1);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipMemcpyAsync(rec, c->records, sizeof(sgk_record), hipMemcpyDeviceToHost, c->stream));
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_rdf_clear(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipMemsetAsync(c->hist, 0, 4 * (size_t)c->dp.sizeHistRdf, c->stream));
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_rdf_accumulate(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    sgk::launch_rdf(c);
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_rdf_get(sgk_ctx *c, unsigned int *hist) {
+    if (!c || !hist) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    SGK_HIP(c, hipMemcpy(hist, c->hist, 4 * (size_t)c->dp.sizeHistRdf, hipMemcpyDeviceToHost));
+    return SGK_OK;
+}
+
+// ---- device-resident loop -----------------------------------------------------------------------------
+// Per-step kernel-sequence flags known on the host: thermostat launch, shear launch, RDF launch.
+static std::string step_flags(sgk_ctx *c, int step0, int nsteps, int rdfEvery) {
+    const SimParams &s = c->sp;
+    std::string f(nsteps, 'a');
+    for (int k = 0; k < nsteps; ++k) {
+        const int step = step0 + k + 1;
+        int b = 0;
+        const bool thermo = s.stepThermostat > 0 && step % s.stepThermostat == 0 &&
+                            ((s.iRegime == 2 && step > s.stepEquil) || s.iRegime == 1);
+        // with deltaF = 0 the shear stays 0 and ApplyShearK adds nothing: no launch needed
+        const bool shear = s.iRegime == 2 && s.nMolMe > 0 && step > s.stepEquil + s.stepCool && s.deltaF != 0.f;
+        const bool rdf = rdfEvery > 0 && step % rdfEvery == 0;
+        b = (thermo ? 1 : 0) | (shear ? 2 : 0) | (rdf ? 4 : 0);
+        f[k] = (char)('a' + b);
+    }
+    return f;
+}
+
+static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
+    sgk::launch_reset_records(c);
+    for (char ch : flags) {
+        const int b = ch - 'a';
+        sgk::launch_lf1(c, true);
+        sgk::launch_nl(c, false);
+        sgk::launch_rho(c);
+        sgk::launch_force(c, true, true);
+        if (b & 1) sgk::launch_thermostat(c, 0, 0.0, 0);
+        if (b & 2) sgk::launch_shear(c);
+        if (b & 4) sgk::launch_rdf(c);
+    }
+}
+
+int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
+    if (!c) return SGK_E_ARG;
+    if (nsteps <= 0) return SGK_OK;
+    if (nsteps > c->cfg.records_capacity)
+        return fail(c, SGK_E_ARG, "nsteps exceeds records_capacity; call sgk_run in chunks");
+    if (c->sp.iRegime == 1) return fail(c, SGK_E_UNSUPPORTED, "regime 1 (deposition) is not implemented by this engine");
+    const std::string flags = step_flags(c, c->hostStep, nsteps, rdfEvery);
+    SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (c->cfg.use_graphs && !c->profiling) {
+        auto it = c->graphs.find(flags);
+        if (it == c->graphs.end()) {
+            if (c->graphs.size() >= 8) {  // bounded cache: drop the least used
+                auto victim = c->graphs.begin();
+                for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
+                    if (g->second.uses < victim->second.uses) victim = g;
+                (void)hipGraphExecDestroy(victim->second.exec);
+                c->graphs.erase(victim);
+            }
+            hipGraph_t graph = nullptr;
+            SGK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            g_capturing = true;
+            enqueue_steps(c, flags);
+            g_capturing = false;
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+            sgk::GraphEntry ge;
+            e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+            it = c->graphs.emplace(flags, ge).first;
+        }
+        it->second.uses += 1;
+        SGK_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
+    } else {
+        enqueue_steps(c, flags);
+    }
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    if (records)
+        SGK_HIP(c, hipMemcpyAsync(records, c->records, sizeof(sgk_record) * (size_t)nsteps, hipMemcpyDeviceToHost,
+                                  c->stream));
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->lastRunMs = ms;
+    if (c->profiling) flush_profile(c);
+    c->hostStep += nsteps;
+    c->haveList = true;
+    return check_flags(c);
+}
+
+double sgk_last_run_ms(sgk_ctx *c) { return c ? c->lastRunMs : 0.0; }
+
+int sgk_set_profiling(sgk_ctx *c, int on) {
+    if (!c) return SGK_E_ARG;
+    c->profiling = on != 0;
+    if (on) c->stats.clear();
+    return SGK_OK;
+}
+
+int sgk_kernel_stats(sgk_ctx *c, sgk_kernel_stat *out, int max) {
+    if (!c) return SGK_E_ARG;
+    flush_profile(c);
+    int n = 0;
+    for (auto &kv : c->stats) {
+        if (out && n < max) {
+            std::memset(&out[n], 0, sizeof(sgk_kernel_stat));
+            std::strncpy(out[n].name, kv.first.c_str(), sizeof(out[n].name) - 1);
+            out[n].launches = kv.second.launches;
+            out[n].ms = kv.second.ms;
+        }
+        ++n;
+    }
+    return n;
+}
+
+long long sgk_device_bytes(sgk_ctx *c) { return c ? c->bytes : 0; }
+
+int sgk_struct_sizes(int *out, int max) {
+    const int s[] = {(int)sizeof(sgk_config), (int)sizeof(sgk_record), (int)sizeof(sgk_kernel_stat)};
+    int n = 0;
+    for (; n < max && n < 3; ++n) out[n] = s[n];
+    return n;
+}
+
+int sgk_metal_lanes(const sgk_ctx *c) { return c ? c->metalLanes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,505 @@
+// sgk_force.hip - EAM density and the force pass for gfx950.
+//
+// The reference evaluates everything in one kernel with one thread per atom and one pass over the
+// neighbour list (ComputeForcesK, SurfaceGrowth.cu:776-1091), after EamComputeRhoK
+// (SurfaceGrowth.cu:716-773).  Here:
+//   k_rho    G lanes per metal atom split its neighbour list; rho, F(rho), F'(rho) stored, so the
+//            force pass reads F'(rho_k) instead of recomputing it per pair.
+//   k_force  one launch: blocks [0, nbM) are the metal pass (G lanes per atom: EAM pairs + LJ with
+//            carbon), blocks [nbM, nbM + nbC) the carbon pass (one thread per atom: bond, plane and
+//            angle terms from the close list + LJ with metal).  The epilogue writes acceleration,
+//            energy, friction, applies leapfrog part 2 and leaves one Partial per block; the last
+//            block to finish reduces them and runs the step bookkeeping (finalize_body).
+// Precision: displacement vectors, cutoff tests and, for the stiff graphene terms, the bond-length
+// and angle deviations are formed in the storage precision R (FP64 in the mixed form).  EAM and LJ
+// pair arithmetic runs in T (FP32, or FP64 with pair_fp64); graphene force-law arithmetic is FP32.
+// Gathers are issued in chunks of predicated loads so several are in flight per thread.
+#include "sgk_device.h"
+#include "sgk_internal.h"
+#include "sgk_reduce.h"
+
+namespace sgk {
+
+// Displacement and its square in storage precision; cutoff decisions are made on this value.
+template <class R, class P4>
+__device__ __forceinline__ R disp(const P4 &a, const P4 &b, const DevParams &p, R &dx, R &dy, R &dz) {
+#pragma clang fp contract(off)
+    dx = wrap1<R>(a.x - b.x, (R)p.Lx);
+    dy = wrap1<R>(a.y - b.y, (R)p.Ly);
+    dz = wrap1<R>(a.z - b.z, (R)p.Lz);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+constexpr int kChunk = 8;  // neighbours gathered per round
+
+// EamComputeRhoK (SurfaceGrowth.cu:716-773) with G lanes per atom; pair math in T.
+template <bool M, class T, int G>
+__global__ __launch_bounds__(256) void k_rho(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                             const int *__restrict__ nlM, const int *__restrict__ nnM,
+                                             float *__restrict__ rho, float *__restrict__ embF,
+                                             float *__restrict__ embDF) {
+    using R = typename Prec<M>::R;
+    using P4 = typename Prec<M>::P4;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / G, l = gt % G;
+    const bool active = i < p.nMolMe;
+    T sum = T(0);
+    if (active) {
+        const P4 A = pos[i];
+        const int nn = nnM[i];
+        const int *row = nlM + (size_t)i * p.nlCap;
+        for (int base = l; base < nn; base += G * 4) {
+            int k[4];
+            P4 B[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = base + q * G;
+                k[q] = j < nn ? row[j] : i;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) B[q] = pos[k[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (k[q] >= p.nMolMe || base + q * G >= nn) continue;
+                R dx, dy, dz;
+                const R rr = disp<R>(A, B[q], p, dx, dy, dz);
+                if (rr < (R)p.rrCutEam) sum += eam_f<T>(p, PM<T>::sqrt((T)rr));
+            }
+        }
+    }
+    sum = group_sum<G>(sum);
+    if (active && l == 0) {
+        T dF;
+        rho[i] = (float)sum;
+        embF[i] = (float)eam_embed<T>(p, sum, dF);
+        embDF[i] = (float)dF;
+    }
+}
+
+// ---- metal pass -----------------------------------------------------------------------------
+// ComputeForcesK metal branch (SurfaceGrowth.cu:856-896, 1069-1082), G lanes per atom; pair math
+// and per-atom sums in T.  Returns the atom's contribution to the block reductions.
+template <bool M, class T, int G>
+__device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
+    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
+    float *__restrict__ acc, const int *__restrict__ nlM, const int *__restrict__ nnM, const float *__restrict__ embF,
+    const float *__restrict__ embDF, float *__restrict__ fric, int integrate) {
+    using R = typename Prec<M>::R;
+    using A = typename Prec<M>::A;
+    using P4 = typename Prec<M>::P4;
+    const int gt = blk * blockDim.x + threadIdx.x;
+    const int i = gt / G, l = gt % G;
+    const bool active = i < p.nMolMe;
+    T fx = T(0), fy = T(0), fz = T(0), frx = T(0), u = T(0);
+    P4 Ai;
+    if (active) {
+        Ai = pos[i];
+        const T dFi = (T)embDF[i];
+        const int nn = nnM[i];
+        const int *row = nlM + (size_t)i * p.nlCap;
+        for (int base = l; base < nn; base += G * 4) {
+            int k[4];
+            P4 B[4];
+            float dFk[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = base + q * G;
+                k[q] = j < nn ? row[j] : i;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                B[q] = pos[k[q]];
+                dFk[q] = k[q] < p.nMolMe ? embDF[k[q]] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (base + q * G >= nn) continue;
+                R dxr, dyr, dzr;
+                const R rrr = disp<R>(Ai, B[q], p, dxr, dyr, dzr);
+                const T dx = (T)dxr, dy = (T)dyr, dz = (T)dzr, rr = (T)rrr;
+                if (k[q] < p.nMolMe) {
+                    if (rrr < (R)p.rrCutEam) {
+                        const T r = PM<T>::sqrt(rr);
+                        T phi, mdphi, df;
+                        eam_pair<T>(p, r, phi, mdphi, df);
+                        const T fc = (mdphi - df * (dFi + (T)dFk[q])) * PM<T>::rcp(r);
+                        fx += fc * dx;
+                        fy += fc * dy;
+                        fz += fc * dz;
+                        u += phi;
+                    }
+                } else if (rrr < (R)p.rrCutLJ) {
+                    T ul;
+                    const T fc = lj_pair<T>(p, rr, ul);
+                    fx += fc * dx;
+                    fy += fc * dy;
+                    fz += fc * dz;
+                    frx += fc * dx;
+                    u += ul;
+                }
+            }
+        }
+    }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    frx = group_sum<G>(frx);
+    u = group_sum<G>(u);
+    Partial<A> q;
+    partial_identity(q);
+    if (active && l == 0) {
+        const float ax = (float)(fx / (T)p.massMe), ay = (float)(fy / (T)p.massMe), az = (float)(fz / (T)p.massMe);
+        acc[i] = ax;
+        acc[p.nMol + i] = ay;
+        acc[2 * p.nMol + i] = az;
+        const T e = T(0.5f) * u + (T)embF[i];
+        pos[i].w = (R)e;
+        fric[i] = (float)frx;
+        R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
+        if (integrate) {
+#pragma clang fp contract(off)
+            const R hdt = (R)(0.5f * p.deltaT);
+            vx = vx + hdt * (R)ax;
+            vy = vy + hdt * (R)ay;
+            vz = vz + hdt * (R)az;
+            vel[i] = vx;
+            vel[p.nMol + i] = vy;
+            vel[2 * p.nMol + i] = vz;
+        }
+        const A m = (A)p.massMe;
+        q.u = (A)e;
+        q.fr = (A)frx;
+        q.com[0] = (A)Ai.x;
+        q.com[1] = (A)Ai.y;
+        q.com[2] = (A)Ai.z;
+        q.cmv[0] = (A)vx;
+        q.cmv[1] = (A)vy;
+        q.cmv[2] = (A)vz;
+        q.vs[0] = m * (A)vx;
+        q.vs[1] = (A)vy;
+        q.vs[2] = (A)vz;
+        const A v2 = (A)vx * (A)vx + (A)vy * (A)vy + (A)vz * (A)vz;
+        q.vv = m * v2;
+        q.vmax = v2;
+        for (int d = 0; d < 3; ++d) q.bmin[d] = q.bmax[d] = q.com[d];
+    }
+    return q;
+}
+
+// ---- carbon pass ----------------------------------------------------------------------------
+// Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form: in FP64
+// through sin(theta - 2pi/3) and cos(theta - 2pi/3), which has no cancellation near the
+// equilibrium angle, plus the constant (2pi/3 - theta_0) because theta_0 is the float 2pi/3 of the
+// reference.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
+template <class R>
+struct AngleDev;
+template <>
+struct AngleDev<float> {
+    __device__ static __forceinline__ float eval(float c, const DevParams &p, float &s) {
+        s = sqrtf(1.f - c * c);
+        return acosf(c) - p.theta_0;
+    }
+};
+template <>
+struct AngleDev<double> {
+    __device__ static __forceinline__ float eval(double c, const DevParams &p, float &s) {
+        const double sd = sqrt(1.0 - c * c);
+        const double h = 0.86602540378443864676;  // sin(2pi/3)
+        const double u = -0.5 * sd - h * c;        // sin(theta - 2pi/3)
+        const double v = -0.5 * c + h * sd;        // cos(theta - 2pi/3)
+        s = (float)sd;
+        return (float)(atan2(u, v) + (2.0943951023931954923 - (double)p.theta_0));
+    }
+};
+
+template <class R>
+struct V3 {
+    R x, y, z;
+};
+
+// Neighbour-angle terms of bond (i,k) with k's bonded neighbour m (SurfaceGrowth.cu:953-978).
+// rik: r_i - r_k, aik = |rik|; e: r_m - r_k, rr2 = |e|^2 (the reference divides by rr2 where |e|
+// is meant; kept).
+template <class R>
+__device__ __forceinline__ void neighbour_angle(const DevParams &p, const V3<R> &rik, R aik, const V3<R> &e, R rr2,
+                                                float mz_rel, float &fx, float &fy, float &fz) {
+    const R c = (rik.x * e.x + rik.y * e.y + rik.z * e.z) / (aik * rr2);
+    float s;
+    const float dth = AngleDev<R>::eval(c, p, s);
+    const float tmp = p.mu_theta * dth / s;
+    const float fa = (float)aik, frr = (float)rr2;
+    float fc = tmp / (fa * frr);
+    fx += fc * (float)e.x;
+    fy += fc * (float)e.y;
+    fz += fc * (float)e.z;
+    fc = -tmp * (float)c / (fa * fa);
+    fx += fc * (float)rik.x;
+    fy += fc * (float)rik.y;
+    fz += fc * (float)rik.z;
+    fz += -p.mu_p * mz_rel / 9.f;
+}
+
+// ComputeForcesK carbon branch (SurfaceGrowth.cu:899-1062, 1083-1088), one thread per carbon.
+template <bool M, class T>
+__device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
+    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
+    float *__restrict__ acc, const unsigned *__restrict__ lj, const int *__restrict__ nlj,
+    const unsigned *__restrict__ cl, const int *__restrict__ ncl, int integrate) {
+    using R = typename Prec<M>::R;
+    using A = typename Prec<M>::A;
+    using P4 = typename Prec<M>::P4;
+    const int ic = blk * blockDim.x + threadIdx.x;
+    Partial<A> q;
+    partial_identity(q);
+    if (ic >= p.nC) return q;
+    const int i = p.nMolMe + ic;
+    const P4 Ai = pos[i];
+    const R z0 = (R)p.z_0;
+    float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
+    V3<R> rnn[3];
+    R absR[3];
+    float dZ[3];
+    int kb[3];
+    P4 Bk[3];
+    int counti = 0, j0 = -1;
+    // Bonds: the first three carbon neighbours, in list order, closer than rCutC (.cu:921).
+    const int nclose = ncl[ic];
+    for (int base = 0; base < nclose && counti < 3; base += kChunk) {
+        unsigned ent[kChunk];
+        P4 B[kChunk];
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t)
+            ent[t] = base + t < nclose ? cl[(size_t)(base + t) * p.nC + ic] : ((unsigned)i << 8);
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) {
+            const int k = (int)(ent[t] >> 8);
+            if (base + t >= nclose || counti >= 3 || k < p.nMolMe) continue;
+            V3<R> rik;
+            const R rrr = disp<R>(Ai, B[t], p, rik.x, rik.y, rik.z);
+            if (!(rrr < (R)p.rrCutC)) continue;
+            rnn[counti] = rik;
+            absR[counti] = sqrt(rrr);
+            dZ[counti] = (float)(B[t].z - z0);
+            kb[counti] = k;
+            Bk[counti] = B[t];
+            ++counti;
+            if (counti == 3) j0 = (int)(ent[t] & 255u);
+        }
+    }
+    // For each bond k: k's own bonded neighbours m != i, the first two in k's list order.
+    for (int b = 0; b < counti; ++b) {
+        const int kc = kb[b] - p.nMolMe;
+        const int nk = ncl[kc];
+        int countk = 0;
+        for (int base = 0; base < nk && countk < 2; base += kChunk) {
+            unsigned ent[kChunk];
+            P4 B[kChunk];
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t)
+                ent[t] = base + t < nk ? cl[(size_t)(base + t) * p.nC + kc] : ((unsigned)i << 8);
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t) {
+                const int m = (int)(ent[t] >> 8);
+                if (base + t >= nk || countk >= 2 || m == i) continue;
+                V3<R> e;
+                const R rr2 = disp<R>(Bk[b], B[t], p, e.x, e.y, e.z);
+                if (!(rr2 < (R)p.rrCutC)) continue;
+                const V3<R> em = {-e.x, -e.y, -e.z};
+                neighbour_angle<R>(p, rnn[b], absR[b], em, rr2, (float)(B[t].z - z0), fx, fy, fz);
+                ++countk;
+            }
+        }
+    }
+    if (counti == 3) {  // SurfaceGrowth.cu:985-1061
+        for (int b = 0; b < 3; ++b) {
+            const float dr = (float)(absR[b] - R(1));
+            const float fc = -p.mu_r * dr / (float)absR[b];
+            fx += fc * (float)rnn[b].x;
+            fy += fc * (float)rnn[b].y;
+            fz += fc * (float)rnn[b].z;
+            u += p.mu_r * dr * dr;
+        }
+        const float zi = (float)(Ai.z - z0);
+        float dz = 2.f * zi - (dZ[0] + dZ[1] + dZ[2]);
+        fz += -0.666666667f * p.mu_p * dz;
+        dz = zi - 0.33333333f * (dZ[0] + dZ[1] + dZ[2]);
+        u += p.mu_p * dz * dz;
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+            const int a = qq, b = qq == 2 ? 0 : qq + 1;
+            const R c = (rnn[a].x * rnn[b].x + rnn[a].y * rnn[b].y + rnn[a].z * rnn[b].z) / (absR[a] * absR[b]);
+            float s;
+            const float dth = AngleDev<R>::eval(c, p, s);
+            const float tmp = p.mu_theta * dth / s;
+            const float ra = (float)absR[a], rb = (float)absR[b], fcos = (float)c;
+            const float inv = 1.f / (ra * rb);
+            const float fcb = tmp * (1.f - ra * fcos / rb) * inv;
+            const float fca = tmp * (1.f - rb * fcos / ra) * inv;
+            fx += fcb * (float)rnn[b].x + fca * (float)rnn[a].x;
+            fy += fcb * (float)rnn[b].y + fca * (float)rnn[a].y;
+            fz += fcb * (float)rnn[b].z + fca * (float)rnn[a].z;
+            u += p.mu_theta * dth * dth;
+        }
+    }
+    // LJ with metal.  The reference reuses its neighbour-loop counter for the bond loop
+    // (SurfaceGrowth.cu:989), so after the third bond (list position j0) the loop resumes at
+    // position 4: entries at positions <= j0 count once, positions 4..j0 twice, and with j0 = 2
+    // position 3 not at all.
+    T gx = (T)fx, gy = (T)fy, gz = (T)fz, gu = (T)u;
+    const int nl = nlj[ic];
+    for (int qq = 0; qq < nl; ++qq) {
+        const unsigned e = lj[(size_t)qq * p.nC + ic];
+        const int k = (int)(e >> 8), pk = (int)(e & 255u);
+        const int mult = j0 < 0 ? 1 : (int)(pk <= j0) + (int)(pk >= 4);
+        if (mult == 0) continue;
+        R dxr, dyr, dzr;
+        const R rrr = disp<R>(Ai, pos[k], p, dxr, dyr, dzr);
+        if (!(rrr < (R)p.rrCutLJ)) continue;
+        T ul;
+        const T fc = lj_pair<T>(p, (T)rrr, ul);
+        for (int t = 0; t < mult; ++t) {
+            gx += fc * (T)dxr;
+            gy += fc * (T)dyr;
+            gz += fc * (T)dzr;
+            gu += ul;
+        }
+    }
+    fx = (float)gx;
+    fy = (float)gy;
+    fz = (float)gz;
+    acc[i] = fx;
+    acc[p.nMol + i] = fy;
+    acc[2 * p.nMol + i] = fz;
+    const float e = (float)(T(0.5f) * gu);
+    pos[i].w = (R)e;
+    R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
+    const bool move = Ai.x != (R)p.xLeft && Ai.x < (R)p.xRight && Ai.y != (R)p.yTop && Ai.y != (R)p.yBottom;
+    if (integrate && move) {
+#pragma clang fp contract(off)
+        const R hdt = (R)(0.5f * p.deltaT);
+        vx = vx + hdt * (R)fx;
+        vy = vy + hdt * (R)fy;
+        vz = vz + hdt * (R)fz;
+        vel[i] = vx;
+        vel[p.nMol + i] = vy;
+        vel[2 * p.nMol + i] = vz;
+    }
+    q.u = (A)e;
+    q.vs[0] = (A)vx;
+    q.vs[1] = (A)vy;
+    q.vs[2] = (A)vz;
+    const A v2 = (A)vx * (A)vx + (A)vy * (A)vy + (A)vz * (A)vz;
+    q.vv = v2;
+    q.vmax = v2;
+    return q;
+}
+
+// One launch for both passes; the last block to finish reduces the partials of all blocks.
+template <bool M, class T, int G>
+__global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
+                                               typename Prec<M>::R *__restrict__ vel, float *__restrict__ acc,
+                                               const int *__restrict__ nlM, const int *__restrict__ nnM,
+                                               const float *__restrict__ embF, const float *__restrict__ embDF,
+                                               float *__restrict__ fric, const unsigned *__restrict__ lj,
+                                               const int *__restrict__ nlj, const unsigned *__restrict__ cl,
+                                               const int *__restrict__ ncl, Partial<typename Prec<M>::A> *part,
+                                               int integrate, DevState *st, sgk_record *records, int recCap) {
+    using A = typename Prec<M>::A;
+    Partial<A> q;
+    if ((int)blockIdx.x < nbM)
+        q = metal_body<M, T, G>(p, blockIdx.x, pos, vel, acc, nlM, nnM, embF, embDF, fric, integrate);
+    else
+        q = carbon_body<M, T>(p, blockIdx.x - nbM, pos, vel, acc, lj, nlj, cl, ncl, integrate);
+    const Partial<A> r = block_reduce_partial(q);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+        last = 0;
+        if (st) {
+            __threadfence();
+            last = atomicAdd(&st->blocksDone, 1u) == gridDim.x - 1;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    finalize_body<A>(p, part, gridDim.x, st, records, recCap, 0);
+    if (threadIdx.x == 0) st->blocksDone = 0;
+}
+
+// ---- launchers ---------------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <bool M, class T>
+static void rho_dispatch(sgk_ctx *c) {
+    using P4 = typename Prec<M>::P4;
+    const int G = c->metalLanes;
+    const unsigned g = blocks_for((long long)c->dp.nMolMe * G, 256);
+    const P4 *pos = (const P4 *)c->pos;
+#define SGK_RHO(GG)                                                                                              \
+    hipLaunchKernelGGL((k_rho<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, pos, c->nlM, c->nnM, c->rho, \
+                       c->embF, c->embDF)
+    switch (G) {
+        case 1: SGK_RHO(1); break;
+        case 2: SGK_RHO(2); break;
+        case 4: SGK_RHO(4); break;
+        case 16: SGK_RHO(16); break;
+        default: SGK_RHO(8); break;
+    }
+#undef SGK_RHO
+}
+
+template <bool M, class T>
+static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
+    using P4 = typename Prec<M>::P4;
+    using R = typename Prec<M>::R;
+    using A = typename Prec<M>::A;
+    const int G = c->metalLanes;
+    const unsigned g = (unsigned)(c->nbM + c->nbC);
+    P4 *pos = (P4 *)c->pos;
+    R *vel = (R *)c->vel;
+    Partial<A> *part = (Partial<A> *)c->partials;
+    DevState *st = finalize ? c->state : nullptr;
+#define SGK_F(GG)                                                                                                \
+    hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, c->acc,    \
+                       c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->cl, c->ncl, part,           \
+                       (int)integrate, st, c->records, c->cfg.records_capacity)
+    switch (G) {
+        case 1: SGK_F(1); break;
+        case 2: SGK_F(2); break;
+        case 4: SGK_F(4); break;
+        case 16: SGK_F(16); break;
+        default: SGK_F(8); break;
+    }
+#undef SGK_F
+}
+
+void launch_rho(sgk_ctx *c) {
+    if (c->dp.nMolMe == 0) return;
+    profile_begin(c);
+    if (!c->mixed)
+        rho_dispatch<false, float>(c);
+    else if (c->cfg.pair_fp64)
+        rho_dispatch<true, double>(c);
+    else
+        rho_dispatch<true, float>(c);
+    profile_end(c, "rho");
+}
+
+void launch_force(sgk_ctx *c, bool integrate, bool finalize) {
+    if (c->nbM + c->nbC == 0) return;
+    profile_begin(c);
+    if (!c->mixed)
+        force_dispatch<false, float>(c, integrate, finalize);
+    else if (c->cfg.pair_fp64)
+        force_dispatch<true, double>(c, integrate, finalize);
+    else
+        force_dispatch<true, float>(c, integrate, finalize);
+    profile_end(c, "force");
+}
+
+}  // namespace sgk

@@ -1,0 +1,96 @@
+// sgk_internal.h - host-side engine context and the kernel launchers of each translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sgk.h"
+#include "sgk_params.h"
+
+namespace sgk {
+
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    unsigned long long uses = 0;
+};
+
+struct KernelStat {
+    int launches = 0;
+    double ms = 0;
+};
+
+}  // namespace sgk
+
+struct sgk_ctx {
+    SimParams sp;
+    sgk_config cfg;
+    sgk::DevParams dp;
+    bool mixed = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // state arrays (positions AoS x,y,z,E; velocity and acceleration SoA)
+    void *pos = nullptr;   // float4 or double4 [nMol]
+    void *vel = nullptr;   // R [3*nMol]
+    float *acc = nullptr;  // [3*nMol]
+    float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;  // [nMolMe]
+    // cells (two buffers, see the rebuild protocol in sgk_nebr.hip)
+    int *cellOf = nullptr;
+    int *cnt[2] = {nullptr, nullptr};
+    int *slots[2] = {nullptr, nullptr};
+    int *nonempty[2] = {nullptr, nullptr};
+    // neighbour structures
+    int *nlM = nullptr, *nnM = nullptr;                 // metal: row-major [nMolMe*nlCap]
+    unsigned int *lj = nullptr, *cl = nullptr;          // carbon: column-major packed (idx<<8|pos)
+    int *nlj = nullptr, *ncl = nullptr;
+    int *fullNN = nullptr, *fullNBL = nullptr;          // reference layout (keep_full_nl)
+    // reductions and run state
+    void *partials = nullptr;
+    sgk::DevState *state = nullptr;
+    sgk_record *records = nullptr;
+    unsigned int *hist = nullptr;
+    sgk::DevState *hostState = nullptr;                 // pinned mirror
+    // launch geometry
+    int block = 256, metalLanes = 8, nbM = 0, nbC = 0, nbProps = 0;
+    long long bytes = 0;
+    bool haveList = false;
+    int hostStep = 0;                                   // host copy of the step counter
+    double lastRunMs = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<std::string, sgk::GraphEntry> graphs;
+    bool profiling = false;
+    std::map<std::string, sgk::KernelStat> stats;
+    std::string err;
+};
+
+namespace sgk {
+
+// sgk_nebr.hip
+void launch_bin(sgk_ctx *c, bool force);
+void launch_sortcells(sgk_ctx *c, bool force);
+void launch_nl(sgk_ctx *c, bool force);
+void launch_commit_rebuild(sgk_ctx *c);
+void launch_rdf(sgk_ctx *c);
+// sgk_force.hip
+void launch_rho(sgk_ctx *c);
+void launch_force(sgk_ctx *c, bool integrate, bool finalize);
+// sgk_step.hip
+void launch_lf1(sgk_ctx *c, bool bin);
+void launch_lf2(sgk_ctx *c);
+void launch_props(sgk_ctx *c);
+void launch_finalize(sgk_ctx *c, int mode);          // 0 step, 1 evaluate only
+void launch_thermostat(sgk_ctx *c, int stepOverride, double vvOverride, int useOverride);
+void launch_shear(sgk_ctx *c);
+void launch_reset_records(sgk_ctx *c);
+
+// Per-kernel timing hook used when profiling is on (sgk_api.hip).
+void profile_begin(sgk_ctx *c);
+void profile_end(sgk_ctx *c, const char *name);
+
+// Partial-record size for the context precision.
+size_t partial_bytes(const sgk_ctx *c);
+
+}  // namespace sgk

@@ -1,0 +1,344 @@
+// sgk_nebr.hip - cell binning, neighbour lists and the RDF histogram for gfx950.
+//
+// Rebuild protocol (two cell buffers, cur = state.cellbuf, nxt = cur ^ 1):
+//   binning     cell index per atom with the reference formula, slot by atomicAdd into nxt; the
+//               first atom of a cell appends it to nxt's non-empty list (bin_atom; in the step loop
+//               it runs inside the leapfrog kernel).
+//   k_nl_cell   one wave per non-empty cell of nxt: the 27 neighbour cells are staged in LDS as
+//               tiles sorted ascending in registers (bitonic), so every atom's list comes out in the
+//               reference order; the step path also zeroes cur's counts for the next rebuild.
+//   commit      cellbuf = nxt; cur's non-empty count = 0 (the step reduction, or k_commit).
+// The API path (sgk_bin) bins with k_bin and sorts the stored cells with k_sortcells.  In the
+// device-resident loop the kernels return at once unless state.nebrNow is set, so a captured step
+// needs no host decision.
+//
+// Cell index, neighbour-list membership and order, and RDF bins are integer results defined by the
+// reference's FP32 arithmetic; both precision forms evaluate them in FP32 on the float-rounded
+// positions.  (The graphene sheet lies exactly on a cell boundary in z by construction, so the
+// cell of every carbon atom - and the list order the carbon force loop depends on - is decided by
+// that rounding.)
+#include "sgk_cells.h"
+#include "sgk_device.h"
+#include "sgk_internal.h"
+
+namespace sgk {
+
+// Offset table of ComputeDefs.h:221-234: offset o -> (s[o/9], s[(o/3)%3], s[o%3]), s = {0,1,-1}.
+__device__ __forceinline__ int off_comp(int v) { return v == 0 ? 0 : (v == 1 ? 1 : -1); }
+
+// Neighbour cell of (ccx,ccy,ccz) for offset o with the cell-wrap shift (SurfaceGrowth.cu:641-674).
+template <class R>
+__device__ __forceinline__ int stencil_cell(const DevParams &p, int ccx, int ccy, int ccz, int o, R &shx, R &shy,
+                                            R &shz) {
+    int mx = ccx + off_comp(o / 9), my = ccy + off_comp((o / 3) % 3), mz = ccz + off_comp(o % 3);
+    shx = shy = shz = R(0);
+    if (mx >= p.cx) {
+        mx = 0;
+        shx = (R)p.Lx;
+    } else if (mx < 0) {
+        mx = p.cx - 1;
+        shx = -(R)p.Lx;
+    }
+    if (my >= p.cy) {
+        my = 0;
+        shy = (R)p.Ly;
+    } else if (my < 0) {
+        my = p.cy - 1;
+        shy = -(R)p.Ly;
+    }
+    if (mz >= p.cz) {
+        mz = 0;
+        shz = (R)p.Lz;
+    } else if (mz < 0) {
+        mz = p.cz - 1;
+        shz = -(R)p.Lz;
+    }
+    return (mz * p.cy + my) * p.cx + mx;
+}
+
+// Float copy of a position (exact in the plain form, rounded in the mixed form).
+template <class P4>
+__device__ __forceinline__ float3 f3(const P4 &q) {
+    return make_float3((float)q.x, (float)q.y, (float)q.z);
+}
+
+// Distance test of the neighbour-list build and RDF: ((A - B) - shift)^2 in FP32 with every
+// operation rounded separately, the canonical evaluation the oracle uses.
+__device__ __forceinline__ float shifted_rr(const float3 &a, const float3 &b, float shx, float shy, float shz) {
+#pragma clang fp contract(off)
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    dx = dx - shx;
+    dy = dy - shy;
+    dz = dz - shz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// BinAtomsIntoCellsK (SurfaceGrowth.cu:547-609) as an O(N) pass (API path).
+template <bool M>
+__global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos, int *__restrict__ cellOf,
+                      CellBufs bufs, DevState *st, int force) {
+    if (!force && !st->nebrNow) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nMol) return;
+    bin_atom(p, i, f3(pos[i]), cellOf, bufs, st);
+}
+
+// Sort each non-empty cell of the new buffer (<= 64 atoms, one per lane) ascending; zero the
+// counts of the outgoing buffer so it is clean for the rebuild after this one.
+__global__ void k_sortcells(DevParams p, CellBufs bufs, DevState *st, int force) {
+    if (!force && !st->nebrNow) return;
+    const int b = st->cellbuf ^ 1, old = st->cellbuf;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const int ne = st->nonempty[b];
+    for (int e = wave; e < ne; e += nwaves) {
+        const int c = bufs.nonempty[b][e];
+        const int n = min(bufs.cnt[b][c], p.cellCap);
+        if (n <= 1) continue;
+        int *s = bufs.slots[b] + (size_t)c * p.cellCap;
+        int v = lane < n ? s[lane] : 0x7fffffff;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int w = __shfl_xor(v, j, 64);
+                const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+                v = (lower == up) ? min(v, w) : max(v, w);
+            }
+        }
+        if (lane < n) s[lane] = v;
+    }
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+    const int neo = st->nonempty[old];
+    for (int e = tid; e < neo; e += nth) bufs.cnt[old][bufs.nonempty[old][e]] = 0;
+}
+
+// Bitonic sort of one value per lane across the wave (64 lanes), ascending.
+__device__ __forceinline__ int wave_sort_asc(int v, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int w = __shfl_xor(v, j, 64);
+            const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+            v = (lower == up) ? min(v, w) : max(v, w);
+        }
+    }
+    return v;
+}
+
+// BuildNebrListK (SurfaceGrowth.cu:612-711), Anderson et al. algorithm 4 as in the reference: one
+// wave per home cell, lane l = l-th atom of the cell; for each of the 27 offset cells in table
+// order the cell is loaded once (one atom per lane), sorted, staged in LDS, and every home atom
+// tests the tile in ascending order.  Outputs:
+//   metal atom:  nlM[i*nlCap + j] = k                     (all neighbours)
+//   carbon atom: lj[q*nC + ic] = k<<8 | j   for metal k   (LJ partners with list position j)
+//                cl[q*nC + ic] = k<<8 | j   if r^2 < closeR2 (bond and angle candidates)
+//   keepFull:    fullNBL[j*nMol + i] = k, fullNN[i]      (reference layout)
+template <bool M>
+__global__ __launch_bounds__(64) void k_nl_cell(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                                CellBufs bufs, DevState *st, int force, int fromNext, int zeroOld,
+                                                int *__restrict__ nlM, int *__restrict__ nnM,
+                                                unsigned *__restrict__ lj, int *__restrict__ nlj,
+                                                unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                                int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    if (!force && !st->nebrNow) return;
+    const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
+    const int *cnt = bufs.cnt[b];
+    const int *slots = bufs.slots[b];
+    const int lane = threadIdx.x;
+    __shared__ int tk[64];
+    __shared__ float tx[64], ty[64], tz[64];
+    const int ne = st->nonempty[b];
+    const float rrN = p.rrNebr, rrClose = p.closeR2;
+    for (int e = blockIdx.x; e < ne; e += gridDim.x) {
+        const int c = bufs.nonempty[b][e];
+        const int nh = min(cnt[c], p.cellCap);
+        const int i = lane < nh ? slots[(size_t)c * p.cellCap + lane] : -1;
+        const float3 A = i >= 0 ? f3(pos[i]) : make_float3(0.f, 0.f, 0.f);
+        const int ccz = c / (p.cx * p.cy);
+        const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
+        const int ccx = c - (ccz * p.cy + ccy) * p.cx;
+        const bool metal = i >= 0 && i < p.nMolMe;
+        const int ic = i - p.nMolMe;
+        int nn = 0, nljv = 0, nclv = 0, err = 0;
+        for (int o = 0; o < 27; ++o) {
+            float shx, shy, shz;
+            const int C = stencil_cell<float>(p, ccx, ccy, ccz, o, shx, shy, shz);
+            const int nt = min(cnt[C], p.cellCap);
+            int v = lane < nt ? slots[(size_t)C * p.cellCap + lane] : 0x7fffffff;
+            v = wave_sort_asc(v, lane);
+            const float3 B = lane < nt ? f3(pos[v]) : make_float3(0.f, 0.f, 0.f);
+            __syncthreads();
+            tk[lane] = v;
+            tx[lane] = B.x;
+            ty[lane] = B.y;
+            tz[lane] = B.z;
+            __syncthreads();
+            if (i < 0) continue;
+            for (int t = 0; t < nt; ++t) {
+                const int k = tk[t];
+                const float rr = shifted_rr(A, make_float3(tx[t], ty[t], tz[t]), shx, shy, shz);
+                if (!(rr < rrN && k != i)) continue;
+                if (nn < p.nlCap) {
+                    if (p.keepFull) fullNBL[(size_t)nn * p.nMol + i] = k;
+                    if (metal) {
+                        nlM[(size_t)i * p.nlCap + nn] = k;
+                    } else {
+                        const unsigned ent = ((unsigned)k << 8) | (unsigned)nn;
+                        if (k < p.nMolMe) {
+                            if (nljv < p.ljCap)
+                                lj[(size_t)nljv * p.nC + ic] = ent;
+                            else
+                                err |= SGK_FLAG_LJ_OVERFLOW;
+                            ++nljv;
+                        }
+                        if (rr < rrClose) {
+                            if (nclv < p.clCap)
+                                cl[(size_t)nclv * p.nC + ic] = ent;
+                            else
+                                err |= SGK_FLAG_CLOSE_OVERFLOW;
+                            ++nclv;
+                        }
+                    }
+                } else {
+                    err |= SGK_FLAG_NL_OVERFLOW;
+                }
+                ++nn;
+            }
+        }
+        if (i >= 0) {
+            nn = min(nn, p.nlCap);
+            if (p.keepFull) fullNN[i] = nn;
+            if (metal) {
+                nnM[i] = nn;
+            } else {
+                nlj[ic] = min(nljv, p.ljCap);
+                ncl[ic] = min(nclv, p.clCap);
+            }
+            if (err) atomicOr(&st->errors, err);
+        }
+    }
+    if (zeroOld) {  // the outgoing buffer becomes the spare: clear its counts for the next rebuild
+        const int old = b ^ 1;
+        const int neo = st->nonempty[old];
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < neo; e += gridDim.x * blockDim.x)
+            bufs.cnt[old][bufs.nonempty[old][e]] = 0;
+    }
+}
+
+// Make the freshly built buffer current (API path; the step path does this in its reduction).
+__global__ void k_commit(DevState *st) {
+    const int old = st->cellbuf;
+    st->cellbuf = old ^ 1;
+    st->nonempty[old] = 0;
+    st->rebuilds += 1;
+}
+
+// EvalRdfK (SurfaceGrowth.cu:1895-1999): metal-metal pairs of the 27-cell stencil of the current
+// cells, r^2 < rangeRdf^2, bin = int(sqrt(r^2) * intervalRdf).  One thread per metal atom.
+// Histogram in LDS, merged with one atomic per bin and block; counts are order independent.
+template <bool M>
+__global__ __launch_bounds__(256) void k_rdf(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                             const int *__restrict__ cellOf, CellBufs bufs, DevState *st,
+                                             unsigned *__restrict__ hist) {
+    extern __shared__ unsigned sh[];
+    for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = i < p.nMolMe ? cellOf[i] : -1;
+    int edge = 0;
+    if (c >= 0) {
+        const int cb = st->cellbuf;
+        const int *cnt = bufs.cnt[cb];
+        const int *slots = bufs.slots[cb];
+        const int ccz = c / (p.cx * p.cy);
+        const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
+        const int ccx = c - (ccz * p.cy + ccy) * p.cx;
+        const float3 A = f3(pos[i]);
+        const float range2 = p.rangeRdf2;
+        for (int o = 0; o < 27; ++o) {
+            float shx, shy, shz;
+            const int C = stencil_cell<float>(p, ccx, ccy, ccz, o, shx, shy, shz);
+            const int n = min(cnt[C], p.cellCap);
+            const int *s = slots + (size_t)C * p.cellCap;
+            for (int t = 0; t < n; ++t) {
+                const int k = s[t];
+                if (k >= p.nMolMe) continue;
+                const float rr = shifted_rr(A, f3(pos[k]), shx, shy, shz);
+                if (rr < range2 && k != i) {
+                    const int bin = (int)(sqrtf(rr) * p.intervalRdf);  // correctly rounded sqrt
+                    if (bin >= p.sizeHistRdf)
+                        edge = 1;
+                    else
+                        atomicAdd(&sh[bin], 1u);
+                }
+            }
+        }
+    }
+    if (edge) atomicOr(&st->errors, SGK_FLAG_RDF_EDGE);
+    __syncthreads();
+    for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x)
+        if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+// ---- launchers ---------------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_bin(sgk_ctx *c, bool force) {
+    if (c->dp.nMol == 0) return;
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_bin<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, (int)force);
+    else
+        hipLaunchKernelGGL(k_bin<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, (int)force);
+    profile_end(c, "bin");
+}
+
+void launch_sortcells(sgk_ctx *c, bool force) {
+    profile_begin(c);
+    const int waves = std::min<long long>(std::max(1, c->dp.nMol), 16384);
+    const unsigned g = blocks_for((long long)waves * 64, 256);
+    hipLaunchKernelGGL(k_sortcells, dim3(g), dim3(256), 0, c->stream, c->dp, cell_bufs(c), c->state, (int)force);
+    profile_end(c, "sortcells");
+}
+
+void launch_nl(sgk_ctx *c, bool force) {
+    if (c->dp.nMol == 0) return;
+    profile_begin(c);
+    const unsigned g = (unsigned)std::min(c->dp.ncell, 8192);
+    const int step = force ? 0 : 1;  // step path: build from the new buffer, clear the old one
+    if (c->mixed)
+        hipLaunchKernelGGL(k_nl_cell<true>, dim3(g), dim3(64), 0, c->stream, c->dp, (const double4 *)c->pos,
+                           cell_bufs(c), c->state, (int)force, step, step, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
+                           c->ncl, c->fullNN, c->fullNBL);
+    else
+        hipLaunchKernelGGL(k_nl_cell<false>, dim3(g), dim3(64), 0, c->stream, c->dp, (const float4 *)c->pos,
+                           cell_bufs(c), c->state, (int)force, step, step, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
+                           c->ncl, c->fullNN, c->fullNBL);
+    profile_end(c, "nl");
+}
+
+void launch_commit_rebuild(sgk_ctx *c) {
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, c->stream, c->state);
+}
+
+void launch_rdf(sgk_ctx *c) {
+    if (c->dp.nMolMe == 0) return;
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMolMe, 256);
+    const size_t shm = sizeof(unsigned) * (size_t)c->dp.sizeHistRdf;
+    if (c->mixed)
+        hipLaunchKernelGGL(k_rdf<true>, dim3(g), dim3(256), shm, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, c->hist);
+    else
+        hipLaunchKernelGGL(k_rdf<false>, dim3(g), dim3(256), shm, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, c->hist);
+    profile_end(c, "rdf");
+}
+
+}  // namespace sgk

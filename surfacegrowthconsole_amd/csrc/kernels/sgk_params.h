@@ -1,0 +1,60 @@
+// sgk_params.h - run constants passed by value to every kernel (kernel-argument segment), and the
+// device-resident run state.  Plain structs usable from host and device code.
+#pragma once
+
+#include <stdint.h>
+
+namespace sgk {
+
+struct DevParams {
+    int nMol, nMolMe, nC, iRegime;
+    float Lx, Ly, Lz;          // region
+    int cx, cy, cz, ncell;     // cell grid
+    float iwx, iwy, iwz;       // invWidth
+    float rrNebr, rrCutEam, rrCutLJ, rrCutC, closeR2;
+    // EAM
+    float re, fe, rhoe, alpha, beta, A, B, kappa, lambda, Fn[4], F[4], eta, Fe, massMe, rei, rhoei;
+    // graphene
+    float mu_r, theta_0, mu_theta, mu_p, z_0;
+    // LJ
+    float epsLJ, sigma2, inv_sigma2;
+    // integration / thermostat
+    float deltaT, temperature, kB, gamma, rNebrShell;
+    int stepThermostat, stepEquil, stepCool;
+    float xLeft, xRight, yBottom, yTop;  // frozen graphene perimeter (SurfaceGrowth.cu:415-424)
+    // RDF
+    float rangeRdf2, intervalRdf;
+    int sizeHistRdf;
+    // shear
+    float deltaF;
+    // capacities and policies
+    int cellCap, nlCap, ljCap, clCap, keepFull;
+    int rebuildEvery;
+    float dispBound;           // max dispHi for which the carbon close lists are exact
+};
+
+// Per-block partial reductions written by the force kernels (or the props kernel).
+template <class T>
+struct Partial {
+    T u, fr, com[3], cmv[3], vs[3], vv, vmax, bmin[3], bmax[3];
+};
+
+// Device-resident run state (one per context).
+struct DevState {
+    int step;              // stepCount after the last finished step
+    int nebrNow;           // rebuild at the next step
+    int errors;            // SGK_FLAG_* bits
+    int cellbuf;           // cell buffer holding the current lists (0/1)
+    int nonempty[2];       // number of entries in each buffer's non-empty-cell list
+    int rec;               // next record slot
+    int rebuilds;          // rebuild counter
+    double dispHi;
+    double shear;
+    double vvSum;          // of the last step (thermostat input)
+    double comX;           // metal centre of mass x of the last step (shear input)
+    double cmVelX;         // metal centre-of-mass velocity x of the last step
+    unsigned int shearCount;
+    unsigned int blocksDone;  // force-kernel blocks finished in this step (last-block reduction)
+};
+
+}  // namespace sgk

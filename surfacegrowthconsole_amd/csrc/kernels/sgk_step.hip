@@ -1,0 +1,250 @@
+// sgk_step.hip - integrator, thermostat, shear, reductions and the per-step bookkeeping.
+//
+// The reference copies nine reduction results to the host every step and decides on the host
+// whether to rebuild the neighbour list (SurfaceGrowth.cu:2242-2395).  Here the force kernels leave
+// one Partial per block and k_finalize (one workgroup) reduces them in a fixed order, evaluates the
+// same host arithmetic on the device, updates dispHi / nebrNow / shear and appends an sgk_record,
+// so consecutive steps run without any host round trip.
+#include "sgk_cells.h"
+#include "sgk_device.h"
+#include "sgk_internal.h"
+#include "sgk_reduce.h"
+
+namespace sgk {
+
+// LeapfrogStepK part 1 (SurfaceGrowth.cu:397-434) fused with ApplyBoundaryCondK (436-453) and,
+// when the device flag asks for a rebuild, with the cell binning of the new position
+// (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
+template <bool M>
+__global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
+                                             typename Prec<M>::R *__restrict__ vel, const float *__restrict__ acc,
+                                             int *__restrict__ cellOf, CellBufs bufs, DevState *st, int doBin) {
+#pragma clang fp contract(off)
+    using R = typename Prec<M>::R;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMol) return;
+    auto r = pos[n];
+    bool move = true;
+    if (n >= p.nMolMe)
+        move = r.x != (R)p.xLeft && r.x < (R)p.xRight && r.y != (R)p.yTop && r.y != (R)p.yBottom;
+    if (move) {
+        const R hdt = (R)(0.5f * p.deltaT), dt = (R)p.deltaT;
+        R vx = vel[n], vy = vel[p.nMol + n], vz = vel[2 * p.nMol + n];
+        vx = vx + hdt * (R)acc[n];
+        vy = vy + hdt * (R)acc[p.nMol + n];
+        vz = vz + hdt * (R)acc[2 * p.nMol + n];
+        vel[n] = vx;
+        vel[p.nMol + n] = vy;
+        vel[2 * p.nMol + n] = vz;
+        r.x = r.x + dt * vx;
+        r.y = r.y + dt * vy;
+        r.z = r.z + dt * vz;
+    }
+    const R Lx = (R)p.Lx, Ly = (R)p.Ly, Lz = (R)p.Lz;
+    if (r.x >= R(0.5f) * Lx) r.x -= Lx;
+    else if (r.x < -R(0.5f) * Lx) r.x += Lx;
+    if (r.y >= R(0.5f) * Ly) r.y -= Ly;
+    else if (r.y < -R(0.5f) * Ly) r.y += Ly;
+    if (r.z >= R(0.5f) * Lz) r.z -= Lz;
+    else if (r.z < -R(0.5f) * Lz) r.z += Lz;
+    pos[n] = r;
+    if (doBin && st->nebrNow) bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), cellOf, bufs, st);
+}
+
+// LeapfrogStepK part 2 alone (the step loop fuses it into the force kernels).
+template <bool M>
+__global__ __launch_bounds__(256) void k_lf2(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                             typename Prec<M>::R *__restrict__ vel, const float *__restrict__ acc) {
+#pragma clang fp contract(off)
+    using R = typename Prec<M>::R;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMol) return;
+    const auto r = pos[n];
+    if (n >= p.nMolMe && !(r.x != (R)p.xLeft && r.x < (R)p.xRight && r.y != (R)p.yTop && r.y != (R)p.yBottom)) return;
+    const R hdt = (R)(0.5f * p.deltaT);
+    vel[n] = vel[n] + hdt * (R)acc[n];
+    vel[p.nMol + n] = vel[p.nMol + n] + hdt * (R)acc[p.nMol + n];
+    vel[2 * p.nMol + n] = vel[2 * p.nMol + n] + hdt * (R)acc[2 * p.nMol + n];
+}
+
+// Per-atom contributions of the current state (the reductions of SurfaceGrowth.cu:1101-1891).
+template <bool M>
+__global__ __launch_bounds__(256) void k_props(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                               const typename Prec<M>::R *__restrict__ vel,
+                                               const float *__restrict__ fric, Partial<typename Prec<M>::A> *part) {
+    using A = typename Prec<M>::A;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    Partial<A> q;
+    partial_identity(q);
+    if (n < p.nMol) {
+        const auto r = pos[n];
+        const A vx = (A)vel[n], vy = (A)vel[p.nMol + n], vz = (A)vel[2 * p.nMol + n];
+        const A v2 = vx * vx + vy * vy + vz * vz;
+        q.u = (A)r.w;
+        q.vmax = v2;
+        if (n < p.nMolMe) {
+            const A m = (A)p.massMe;
+            q.fr = fric ? (A)fric[n] : A(0);
+            q.com[0] = (A)r.x;
+            q.com[1] = (A)r.y;
+            q.com[2] = (A)r.z;
+            q.cmv[0] = vx;
+            q.cmv[1] = vy;
+            q.cmv[2] = vz;
+            q.vs[0] = m * vx;
+            q.vs[1] = vy;
+            q.vs[2] = vz;
+            q.vv = m * v2;
+            for (int d = 0; d < 3; ++d) q.bmin[d] = q.bmax[d] = q.com[d];
+        } else {
+            q.vs[0] = vx;
+            q.vs[1] = vy;
+            q.vs[2] = vz;
+            q.vv = v2;
+        }
+    }
+    const Partial<A> r = block_reduce_partial(q);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Standalone reduction of the partials (the step loop runs finalize_body inside k_force).
+template <bool M>
+__global__ __launch_bounds__(256) void k_finalize(DevParams p, const Partial<typename Prec<M>::A> *__restrict__ part,
+                                                  int nParts, DevState *st, sgk_record *records, int recCap,
+                                                  int mode) {
+    finalize_body<typename Prec<M>::A>(p, part, nParts, st, records, recCap, mode);
+}
+
+// ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485), kinetic energy of all atoms.
+template <bool M>
+__global__ __launch_bounds__(256) void k_thermostat(DevParams p, typename Prec<M>::R *__restrict__ vel,
+                                                    const DevState *st, int stepOverride, double vvOverride,
+                                                    int useOverride, int metalAll) {
+    using R = typename Prec<M>::R;
+    using A = typename Prec<M>::A;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMol) return;
+    const int step = useOverride ? stepOverride : st->step;
+    const A vv = (A)(useOverride ? vvOverride : st->vvSum);
+    const bool carbonOn = (p.iRegime == 2 && step > p.stepEquil) || p.iRegime == 1 || useOverride;
+    const bool metalOn = p.iRegime == 2 && step > p.stepEquil && (metalAll || step < p.stepEquil + p.stepCool);
+    if (!carbonOn) return;
+    const A ke = vv * A(0.5f) / (A)p.nMol;
+    A b;
+    if (n >= p.nMolMe)
+        b = sqrt(A(1.f) + (A)p.gamma * ((A)p.temperature * A(1.5f) * (A)p.kB / ke - A(1.f)));
+    else if (metalOn)
+        b = sqrt(A(1.f) + (A)p.gamma * ((A)p.temperature * (A)p.kB / ke - A(1.f)));
+    else
+        return;
+    vel[n] = (R)(b * (A)vel[n]);
+    vel[p.nMol + n] = (R)(b * (A)vel[p.nMol + n]);
+    vel[2 * p.nMol + n] = (R)(b * (A)vel[2 * p.nMol + n]);
+}
+
+// ApplyShearK (SurfaceGrowth.cu:517-534): metal atoms left of the centre of mass get the shear
+// acceleration; the count goes into the step's record.
+template <bool M>
+__global__ __launch_bounds__(256) void k_shear(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                               float *__restrict__ acc, const DevState *st, sgk_record *records,
+                                               int recCap) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    int hit = 0;
+    if (n < p.nMolMe) {
+        const float dr = (float)st->comX - (float)pos[n].x;
+        if (dr > 0.f) {
+            acc[n] += (float)st->shear;
+            hit = 1;
+        }
+    }
+    const int cnt = __syncthreads_count(hit);
+    const int slot = st->rec - 1;
+    if (threadIdx.x == 0 && cnt && slot >= 0 && slot < recCap) atomicAdd(&records[slot].shearCount, (double)cnt);
+}
+
+__global__ void k_reset_records(DevState *st) { st->rec = 0; }
+
+// ---- launchers ---------------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+size_t partial_bytes(const sgk_ctx *c) { return c->mixed ? sizeof(Partial<double>) : sizeof(Partial<float>); }
+
+void launch_lf1(sgk_ctx *c, bool bin) {
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_lf1<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double4 *)c->pos, (double *)c->vel,
+                           c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+    else
+        hipLaunchKernelGGL(k_lf1<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos, (float *)c->vel,
+                           c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+    profile_end(c, "lf1");
+}
+
+void launch_lf2(sgk_ctx *c) {
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_lf2<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos,
+                           (double *)c->vel, c->acc);
+    else
+        hipLaunchKernelGGL(k_lf2<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos,
+                           (float *)c->vel, c->acc);
+    profile_end(c, "lf2");
+}
+
+void launch_props(sgk_ctx *c) {
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    float *fric = (c->dp.iRegime != 0 && c->dp.nMolMe) ? c->fric : nullptr;
+    if (c->mixed)
+        hipLaunchKernelGGL(k_props<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos,
+                           (const double *)c->vel, fric, (Partial<double> *)c->partials);
+    else
+        hipLaunchKernelGGL(k_props<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos,
+                           (const float *)c->vel, fric, (Partial<float> *)c->partials);
+    profile_end(c, "props");
+}
+
+void launch_finalize(sgk_ctx *c, int mode) {
+    profile_begin(c);
+    const int nParts = mode == 1 ? c->nbProps : c->nbM + c->nbC;
+    const int cap = mode == 1 ? 1 : c->cfg.records_capacity;
+    if (c->mixed)
+        hipLaunchKernelGGL(k_finalize<true>, dim3(1), dim3(256), 0, c->stream, c->dp,
+                           (const Partial<double> *)c->partials, nParts, c->state, c->records, cap, mode);
+    else
+        hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(256), 0, c->stream, c->dp,
+                           (const Partial<float> *)c->partials, nParts, c->state, c->records, cap, mode);
+    profile_end(c, "finalize");
+}
+
+void launch_thermostat(sgk_ctx *c, int stepOverride, double vvOverride, int useOverride) {
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    const int metalAll = c->cfg.metal_thermostat_all;
+    if (c->mixed)
+        hipLaunchKernelGGL(k_thermostat<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double *)c->vel, c->state,
+                           stepOverride, vvOverride, useOverride, metalAll);
+    else
+        hipLaunchKernelGGL(k_thermostat<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float *)c->vel, c->state,
+                           stepOverride, vvOverride, useOverride, metalAll);
+    profile_end(c, "thermostat");
+}
+
+void launch_shear(sgk_ctx *c) {
+    if (c->dp.nMolMe == 0) return;
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMolMe, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_shear<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->acc,
+                           c->state, c->records, c->cfg.records_capacity);
+    else
+        hipLaunchKernelGGL(k_shear<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->acc,
+                           c->state, c->records, c->cfg.records_capacity);
+    profile_end(c, "shear");
+}
+
+void launch_reset_records(sgk_ctx *c) { hipLaunchKernelGGL(k_reset_records, dim3(1), dim3(1), 0, c->stream, c->state); }
+
+}  // namespace sgk

@@ -1,0 +1,244 @@
+"""GPU parity: every stage of the HIP engine against the CPU oracle on the same inputs.
+
+Integer results (cell index, cell lists, neighbour list and its order, carbon lists, RDF histogram)
+must be bit-exact.  Floating point uses the force-error metric
+e_F = max_i |F_i - F_i^ref| / rms_j |F_j^64| on the same positions and neighbour list: the mixed form
+against the FP64 oracle (<= 1e-5), the plain FP32 form against the FP32 restatement (<= 2e-4).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import surfacegrowthconsole_amd as sg
+
+pytestmark = pytest.mark.gpu
+
+_CACHE = {}
+
+
+def system(name):
+    if name not in _CACHE:
+        cfg, variant = {"small": ("small", "reference"), "icosa": ("icosa", "config"),
+                        "medium": ("medium", "config")}[name]
+        print(f"[parity] building {name}", flush=True)
+        _CACHE[name] = sg.config_system(cfg, variant)
+    return _CACHE[name]
+
+
+def thermalised(name, steps=200):
+    """State after `steps` FP64-oracle steps, so forces are away from the perfect lattice."""
+    key = (name, "hot", steps)
+    if key not in _CACHE:
+        s = system(name)
+        o = oracle.Oracle(s.params, "f64")
+        err, r, v, a, _ = o.run(s.r, s.v, s.a, steps)
+        assert err == 0
+        _CACHE[key] = (r, v, a)
+    return _CACHE[key]
+
+
+def force_error(acc, ref):
+    rms = np.sqrt((ref ** 2).sum(1).mean())
+    return np.abs(acc - ref).max() / rms
+
+
+@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_cells_and_neighbour_list_bit_exact(gpu, name, prec):
+    s = system(name)
+    p = s.params
+    o = oracle.Oracle(p, "f32")
+    print(f"[parity] cells/NL {name} {prec}", flush=True)
+    with sg.Engine(p, precision=prec, keep_full_nl=True) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.bin()
+        cell_of, cnt, slots = eng.get_cells()
+        err, ocell, ocnt, oslots = o.bin(s.r)
+        assert err == 0
+        assert np.array_equal(cell_of, ocell)
+        assert np.array_equal(cnt, ocnt)
+        assert np.array_equal(slots, oslots)
+        eng.build_nebr()
+        nn, nbl = eng.get_nebr()
+        err, onn, onbl = o.nebr(s.r, ocnt, oslots)
+        assert err == 0
+        assert np.array_equal(nn, onn)
+        assert np.array_equal(nbl, onbl)
+        if p.nMol > p.nMolMe:
+            check_carbon_lists(eng, p, s.r.astype(np.float32), onn, onbl)
+
+
+def check_carbon_lists(eng, p, r, nn, nbl):
+    """Carbon lists = the metal entries of the reference list, and the entries closer than the close
+    radius, each with its list position, in list order."""
+    cl = eng.carbon_lists()
+    N, nMe = p.nMol, p.nMolMe
+    nbl = nbl.reshape(-1, N)
+    margin = eng.cfg.close_margin
+    close_r = np.float32(p.rCutC + 2.0 * (0.5 * p.rNebrShell + margin))
+    close2 = np.float32(close_r * close_r)
+    L = np.array([p.region.x, p.region.y, p.region.z], np.float32)
+    rng = np.random.default_rng(0)
+    for i in rng.choice(np.arange(nMe, N), size=min(400, N - nMe), replace=False):
+        ic = i - nMe
+        ks = nbl[: nn[i], i]
+        pos = np.arange(nn[i])
+        metal = ks < nMe
+        assert np.array_equal(cl["lj_idx"][ic, : cl["n_lj"][ic]], ks[metal])
+        assert np.array_equal(cl["lj_pos"][ic, : cl["n_lj"][ic]], pos[metal])
+        d = r[i, :3] - r[ks, :3]
+        d = np.where(d >= np.float32(0.5) * L, d - L, np.where(d < -np.float32(0.5) * L, d + L, d))
+        rr = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        close = rr < close2
+        assert np.array_equal(cl["close_idx"][ic, : cl["n_close"][ic]], ks[close])
+        assert np.array_equal(cl["close_pos"][ic, : cl["n_close"][ic]], pos[close])
+
+
+@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
+@pytest.mark.parametrize("state", ["initial", "thermalised"])
+def test_forces_vs_oracle(gpu, name, state):
+    """Mixed form against the FP64 oracle (e_F <= 1e-5).  Plain form against the FP32 restatement
+    of the same arithmetic (e_F <= 2e-4: FP32 acosf of two libms may differ by 1 ulp, which the angle
+    force amplifies by mu_theta/sin(theta) ~ 110).  The reference forms x_i - x_k before the periodic wrap, so
+    in FP32 a bond across the boundary of a box of edge L carries an error ~ulp(L) times the bond
+    stiffness - a property of the reference algorithm, reported here against FP64, not a defect."""
+    s = system(name)
+    p = s.params
+    if state == "initial":
+        r = s.r.astype(np.float64)
+    else:
+        r = thermalised(name, 100 if name == "medium" else 200)[0]
+    o64 = oracle.Oracle(p, "f64")
+    o32 = oracle.Oracle(p, "f32")
+    for prec in ("fp32", "mixed"):
+        # the plain form stores float positions: its references are evaluated at those positions
+        rin = r if prec == "mixed" else r.astype(np.float32)
+        err, nn, nbl = o64.neighbour_list(rin)
+        assert err == 0
+        rho64 = o64.rho(rin, nn, nbl)
+        a64, e64, f64 = o64.forces(rin, nn, nbl, rho64)
+        print(f"[parity] forces {name} {state} {prec}", flush=True)
+        with sg.Engine(p, precision=prec) as eng:
+            eng.upload(rin)
+            eng.bin()
+            eng.build_nebr()
+            eng.compute_forces()
+            rho = eng.rho()
+            acc, e, fr = eng.forces()
+        ef64 = force_error(acc, a64)
+        if prec == "mixed":
+            print(f"[parity]   e_F vs FP64 = {ef64:.3e}", flush=True)
+            assert ef64 < 1e-5, (name, state, ef64)
+            if p.nMolMe:
+                assert np.abs(rho - rho64).max() / np.abs(rho64).max() < 1e-6
+            assert abs(e.sum() - e64.sum()) <= 1e-7 * np.abs(e64).sum()
+            if p.nMolMe:
+                assert np.abs(fr - f64).max() <= 1e-6 * max(np.abs(f64).max(), 1e-3)
+        else:
+            a32, e32, f32 = o32.forces(rin, nn, nbl)
+            ef32 = np.abs(acc - a32).max() / np.sqrt((a64 ** 2).sum(1).mean())
+            print(f"[parity]   e_F vs FP32 restatement = {ef32:.3e}; vs FP64 = {ef64:.3e}", flush=True)
+            assert ef32 < 2e-4, (name, state, ef32)
+            if p.nMolMe:
+                rho32 = o32.rho(rin, nn, nbl)
+                assert np.abs(rho - rho32).max() / np.abs(rho32).max() < 1e-5
+            assert abs(e.sum() - e32.sum()) <= 1e-5 * np.abs(e32).sum()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_integrator_thermostat_props(gpu, prec):
+    s = system("small")
+    p = s.params
+    r, v, a = thermalised("small", 100)
+    orc = oracle.Oracle(p, "f32" if prec == "fp32" else "f64")
+    if prec == "fp32":
+        r, v, a = r.astype(np.float32), v.astype(np.float32), a.astype(np.float32)
+    else:  # both forms store accelerations in FP32
+        a = a.astype(np.float32).astype(np.float64)
+    with sg.Engine(p, precision=prec) as eng:
+        eng.upload(r, v, a)
+        eng.leapfrog(1)
+        gr, gv, _ = eng.download()
+        orr, ov = orc.leapfrog(1, r, v, a)
+        # leapfrog and the boundary wrap are restated operation for operation: bit-exact
+        assert np.array_equal(gr[:, :3], orr[:, :3].astype(np.float64))
+        assert np.array_equal(gv, ov.astype(np.float64))
+        eng.leapfrog(2)
+        gr, gv, _ = eng.download()
+        _, ov2 = orc.leapfrog(2, orr, ov, a)
+        assert np.array_equal(gv, ov2.astype(np.float64))
+        # reductions with the reference semantics
+        rec = eng.props()
+        opr = orc.props(orr, ov2)
+        for k in ("vvSum", "uSum"):
+            assert abs(rec[k] - opr[k]) <= 1e-5 * max(abs(opr[k]), 1.0), k
+        assert abs(rec["vvMax"] - opr["vvMax"]) <= 1e-6 * opr["vvMax"]
+        for k in ("vSum", "comSum", "cmVelSum"):
+            assert np.allclose(rec[k], opr[k], rtol=1e-5, atol=1e-3), k
+        for k in ("bboxMin", "bboxMax"):
+            assert np.array_equal(np.float32(rec[k]), np.float32(opr[k])), k
+        # thermostat with the same vvSum
+        vv = opr["vvSum"]
+        eng.thermostat(30, vv)
+        _, gv3, _ = eng.download()
+        ov3 = orc.thermostat(ov2, vv, 30)
+        assert np.allclose(gv3, ov3, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_rdf_histogram_bit_exact(gpu, name):
+    s = system(name)
+    p = s.params
+    r = thermalised(name, 50)[0].astype(np.float32)
+    o = oracle.Oracle(p, "f32")
+    err, cell_of, cnt, slots = o.bin(r)
+    hist, edge = o.rdf(r, cell_of, cnt, slots)
+    with sg.Engine(p) as eng:
+        eng.upload(r)
+        eng.bin()
+        eng.rdf_clear()
+        eng.rdf_accumulate()
+        eng.rdf_accumulate()
+        g = eng.rdf()
+    assert hist.sum() > 0
+    assert np.array_equal(g, 2 * hist)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_device_loop_vs_oracle_run(gpu, prec):
+    """sgk_run (device-resident, graph captured) against the oracle's DoComputationsW loop."""
+    s = system("small")
+    p = s.params
+    n = 60
+    o = oracle.Oracle(p, "f64" if prec == "mixed" else "f32")
+    err, r_o, v_o, a_o, orecs = o.run(s.r, s.v, s.a, n)
+    assert err == 0
+    with sg.Engine(p, precision=prec) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.set_run_state(0, 1, 0.0)
+        recs = eng.run(n)
+        r_g, v_g, _ = eng.download()
+    assert [x["step"] for x in recs] == list(range(1, n + 1))
+    assert [x["rebuilt"] for x in recs] == [x["rebuilt"] for x in orecs]
+    tol = 1e-6 if prec == "mixed" else 2e-4
+    for k in ("totEnergy", "potEnergy", "kinEnergy"):
+        a = np.array([x[k] for x in recs])
+        b = np.array([x[k] for x in orecs])
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), k
+    dr = np.abs(r_g[:, :3] - r_o[:, :3]).max()
+    assert dr < (1e-6 if prec == "mixed" else 1e-3), dr
+
+
+def test_graph_and_eager_bitwise_identical(gpu):
+    s = system("small")
+    out = []
+    for graphs in (True, False, True):
+        with sg.Engine(s.params, precision="fp32", use_graphs=graphs) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            eng.run(40)
+            eng.run(40)
+            out.append(eng.download())
+    for x in out[1:]:
+        for a, b in zip(out[0], x):
+            assert np.array_equal(a, b)
