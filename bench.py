@@ -1,0 +1,201 @@
+"""Throughput of the per-step MD engine on one MI355X.
+
+A step is one full MD step of the hot path (leapfrog + boundary + binning when due, neighbour list
+when due, EAM density, fused EAM/LJ/graphene force pass with leapfrog part 2 and the step
+reductions, thermostat on its steps) over the whole system, with all state resident in HBM.
+
+Default workload: the large system - 1,003,520-atom periodic graphene sheet + 100,000-atom Ni fcc
+sphere 2.5 A above it, 1,000 K Maxwell-Boltzmann velocities, dt = 1 fs, Berendsen tau = 0.1 ps,
+neighbour list rebuilt every 10 steps (or sooner by the skin rule).  `value` is the mixed-precision
+form's atom-steps/s; the plain FP32 form is timed on the same workload and reported beside it.
+
+Roofline: the dominant kernel is the force pass; its algorithmic bytes per launch are the
+per-atom-step figure of the state traffic (plain 48 + 20 f_Me B, mixed 96 + 20 f_Me B) times the
+atoms, divided by its mean launch duration from HIP events on the engine stream.
+
+cpu_baseline: the FP32 CPU restatement of the same step (oracle/, single thread) on a bounded
+number of steps of the same workload; its FP64 twin is also the checker for the force error of
+the GPU forms on the initial state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_if_needed():
+    lib = os.path.join(ROOT, "surfacegrowthconsole_amd", "lib", "libsurfacegrowth.so")
+    orc = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not (os.path.exists(lib) and os.path.exists(orc)):
+        import __graft_entry__
+
+        __graft_entry__.build()
+
+
+def b_alg(precision, f_me):
+    return (96.0 if precision == "mixed" else 48.0) + 20.0 * f_me
+
+
+def time_form(sys_, precision, steps, warmup, rebuild, chunk):
+    import surfacegrowthconsole_amd as sg
+
+    p = sys_.params
+    eng = sg.Engine(p, precision=precision, rebuild_every=rebuild, records_capacity=chunk)
+    eng.upload(sys_.r, sys_.v, sys_.a)
+    eng.set_run_state(0, 1, 0.0)
+    done = 0
+    while done < warmup:
+        n = min(chunk, warmup - done)
+        eng.run(n, records=False)
+        done += n
+    # timed region: `steps` device-resident steps; sgk_run synchronises the stream on return
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        eng.run(n, records=False)
+        dev_ms += eng.last_run_ms()
+        done += n
+    wall = time.perf_counter() - t0
+    # per-kernel durations: HIP events around each launch on the engine stream, same step count
+    eng.set_profiling(True)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        eng.run(n, records=False)
+        done += n
+    stats = eng.kernel_stats()
+    eng.set_profiling(False)
+    r, v, a = eng.download()
+    eng.close()
+    return wall, dev_ms, stats
+
+
+def gpu_forces(sys_, precision):
+    import surfacegrowthconsole_amd as sg
+
+    with sg.Engine(sys_.params, precision=precision) as eng:
+        eng.upload(sys_.r)
+        eng.bin()
+        eng.build_nebr()
+        eng.compute_forces()
+        acc, _, _ = eng.forces()
+    return acc
+
+
+def cpu_baseline(sys_, rebuild, seconds, gpu_acc):
+    """FP32 CPU restatement (oracle) timed on the host, single thread, on a bounded sample of steps;
+    the FP64 oracle checks the GPU forces on the initial state."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    p = sys_.params
+    o32 = oracle.Oracle(p, "f32")
+    # one step to size the sample, then ~`seconds` of steps
+    t0 = time.perf_counter()
+    err, r, v, a, _ = o32.run(sys_.r, sys_.v, sys_.a, 1, 0, 1, 0.0, rebuild)
+    t1 = time.perf_counter() - t0
+    n = max(2, min(200, int(seconds / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    err, r, v, a, _ = o32.run(r, v, a, n, 1, 0, 0.0, rebuild)
+    dt = time.perf_counter() - t0
+    rate = p.nMol * n / dt
+    # checker: FP64 oracle forces on the initial state
+    o64 = oracle.Oracle(p, "f64")
+    r0 = sys_.r.astype(np.float64)
+    _, nn, nbl = o64.neighbour_list(r0)
+    a64, _, _ = o64.forces(r0, nn, nbl)
+    rms = float(np.sqrt((a64 ** 2).sum(1).mean()))
+    ferr = {k: float(np.abs(acc - a64).max() / rms) for k, acc in gpu_acc.items()}
+    return {"value": rate, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps (after 1 untimed) of the same workload, FP32 C restatement, 1 thread"}, ferr
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--config", default="large", choices=["small", "medium", "large", "icosa"])
+    ap.add_argument("--variant", default="config", choices=["config", "reference"])
+    ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    build_if_needed()
+    import surfacegrowthconsole_amd as sg
+    from surfacegrowthconsole_amd.configs import rebuild_every
+
+    log(f"[bench] building {args.config}/{args.variant}")
+    s = sg.config_system(args.config, args.variant)
+    p = s.params
+    rebuild = rebuild_every(args.variant)
+    f_me = p.nMolMe / p.nMol
+    res = {}
+    for prec in ("mixed", "fp32"):
+        log(f"[bench] {prec}: {args.warmup} warm-up + {args.steps} timed steps")
+        wall, dev_ms, stats = time_form(s, prec, args.steps, args.warmup, rebuild, args.chunk)
+        res[prec] = dict(wall=wall, dev_ms=dev_ms, stats=stats)
+        log(f"[bench] {prec}: {wall / args.steps * 1e3:.3f} ms/step, {p.nMol * args.steps / wall:.4e} atom-steps/s")
+    m = res["mixed"]
+    value = p.nMol * args.steps / m["wall"]
+    fl, fms = m["stats"]["force"]
+    force_us = fms / fl * 1e3
+    alg_bytes = b_alg("mixed", f_me) * p.nMol
+    achieved = alg_bytes / (force_us * 1e-6) / 1e9
+    kernels = {k: {"launches": v[0], "us_per_launch": round(v[1] / v[0] * 1e3, 3)} for k, v in m["stats"].items()}
+    out = {
+        "metric": "atom-steps/s on 1 MI355X, mixed vs FP32 at <=1e-5 rel force err vs FP64",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state and EAM/LJ pair math, f32 graphene force law",
+        "data": "synthetic (generated lattices, seeded velocities)",
+        "config": {"workload": f"{args.config}: {p.nMol - p.nMolMe} C graphene + {p.nMolMe} Ni fcc sphere, "
+                               f"{p.temperature * p.temperatureU:.0f} K, dt 1 fs, variant {args.variant}",
+                   "atoms": p.nMol, "metal_fraction": round(f_me, 4), "precision": "mixed",
+                   "rebuild_every": rebuild, "parallelism": "single GPU"},
+        "fp32": {"value": p.nMol * args.steps / res["fp32"]["wall"],
+                 "ms_per_step": res["fp32"]["wall"] / args.steps * 1e3},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_force (fused metal+carbon force pass)", "us_per_launch": force_us,
+                     "alg_bytes_per_atom_step": b_alg("mixed", f_me)},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline + force-error check")
+        acc = {"mixed": gpu_forces(s, "mixed"), "fp32": gpu_forces(s, "fp32")}
+        cb, ferr = cpu_baseline(s, rebuild, args.cpu_seconds, acc)
+        out["cpu_baseline"] = cb
+        out["force_error_vs_fp64"] = ferr
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -250,7 +250,9 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         return fail(nullptr, SGK_E_ARG, "invalid atom counts");
     if (sp->nMol >= (1 << 24)) return fail(nullptr, SGK_E_UNSUPPORTED, "more than 2^24 atoms");
     if (cfg.nl_capacity > 255) return fail(nullptr, SGK_E_UNSUPPORTED, "neighbour capacity above 255");
-    if (cfg.cell_capacity > 64) return fail(nullptr, SGK_E_UNSUPPORTED, "cell capacity above 64 (one wave per cell)");
+    if (cfg.cell_capacity > 64 || cfg.cell_capacity % 8)
+        return fail(nullptr, SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64 (one wave per cell)");
+    if (cfg.close_capacity % 8) return fail(nullptr, SGK_E_UNSUPPORTED, "close capacity must be a multiple of 8");
     if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return fail(nullptr, SGK_E_ARG, "empty cell grid");
     if (sp->iRegime < 0 || sp->iRegime > 2) return fail(nullptr, SGK_E_ARG, "regime must be 0, 1 or 2");
     if (sp->sizeHistRdf <= 0 || sp->sizeHistRdf > 8192) return fail(nullptr, SGK_E_ARG, "RDF bins must be 1..8192");
@@ -288,6 +290,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     c->block = 256;
     c->nbM = (int)(((long long)p.nMolMe * G + 255) / 256);
     c->nbC = (p.nC + 255) / 256;
+    c->nbRho = (int)(((long long)p.nMolMe * G + 255) / 256);
     c->nbProps = (p.nMol + 255) / 256;
     const size_t R = c->mixed ? 8 : 4;
     const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
@@ -310,6 +313,8 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     SGK_TRY(dalloc(c, &c->nlj, nC, true));
     SGK_TRY(dalloc(c, &c->cl, nC * (size_t)p.clCap, false));
     SGK_TRY(dalloc(c, &c->ncl, nC, true));
+    SGK_TRY(dalloc(c, &c->bondC, nC, true));
+    SGK_TRY(dalloc(c, &c->bondA, nC, true));
     if (p.keepFull) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
         SGK_TRY(dalloc(c, &c->fullNBL, nMol * (size_t)p.nlCap, false));
@@ -348,7 +353,7 @@ int sgk_destroy(sgk_ctx *c) {
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf, c->cnt[0], c->cnt[1],
                     c->slots[0], c->slots[1], c->nonempty[0], c->nonempty[1], c->nlM, c->nnM, c->lj, c->nlj, c->cl,
-                    c->ncl, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist};
+                    c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->hostState) (void)hipHostFree(c->hostState);
@@ -577,7 +582,7 @@ int sgk_get_carbon_lists(sgk_ctx *c, int *nLj, int *ljIdx, int *ljPos, int *nClo
             if (ljPos) ljPos[ic * c->dp.ljCap + q] = q < nl[ic] ? (int)(e & 255u) : -1;
         }
         for (int q = 0; q < c->dp.clCap; ++q) {
-            const unsigned e = q < nc[ic] ? cl[(size_t)q * nC + ic] : 0xffffffffu;
+            const unsigned e = q < nc[ic] ? cl[ic * (size_t)c->dp.clCap + q] : 0xffffffffu;
             if (closeIdx) closeIdx[ic * c->dp.clCap + q] = q < nc[ic] ? (int)(e >> 8) : -1;
             if (closePos) closePos[ic * c->dp.clCap + q] = q < nc[ic] ? (int)(e & 255u) : -1;
         }
@@ -709,6 +714,7 @@ static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
     for (char ch : flags) {
         const int b = ch - 'a';
         sgk::launch_lf1(c, true);
+        sgk::launch_sortcells(c, false);
         sgk::launch_nl(c, false);
         sgk::launch_rho(c);
         sgk::launch_force(c, true, true);

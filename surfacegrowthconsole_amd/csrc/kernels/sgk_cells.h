@@ -44,9 +44,16 @@ __device__ __forceinline__ void bin_atom(const DevParams &p, int n, float3 q, in
     cellOf[n] = c;
     const int b = st->cellbuf ^ 1;
     const int slot = atomicAdd(&bufs.cnt[b][c], 1);
-    if (slot == 0) {
-        const int e = atomicAdd(&st->nonempty[b], 1);
-        bufs.nonempty[b][e] = c;
+    // Append newly occupied cells with one atomic per wave (a single counter hit once per cell
+    // serialises at the L2).
+    const unsigned long long first = __ballot(slot == 0);
+    if (first) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)first) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&st->nonempty[b], __popcll(first));
+        base = __shfl(base, leader, 64);
+        if (slot == 0) bufs.nonempty[b][base + __popcll(first & ((1ull << lane) - 1ull))] = c;
     }
     if (slot < p.cellCap)
         bufs.slots[b][(size_t)c * p.cellCap + slot] = n;

@@ -3,8 +3,12 @@
 // The reference evaluates everything in one kernel with one thread per atom and one pass over the
 // neighbour list (ComputeForcesK, SurfaceGrowth.cu:776-1091), after EamComputeRhoK
 // (SurfaceGrowth.cu:716-773).  Here:
-//   k_rho    G lanes per metal atom split its neighbour list; rho, F(rho), F'(rho) stored, so the
-//            force pass reads F'(rho_k) instead of recomputing it per pair.
+//   k_rho    blocks [0, nbRho): G lanes per metal atom split its neighbour list; rho, F(rho),
+//            F'(rho) stored, so the force pass reads F'(rho_k) instead of recomputing it per pair.
+//            blocks [nbRho, nbRho + nbC): one thread per carbon scans its close list once and stores
+//            the first three carbons within rCutC (its bonds) and the first three atoms of any kind
+//            within rCutC (what a bonded neighbour's angle terms need), so the force pass reads
+//            four near lists instead of scanning four close lists.
 //   k_force  one launch: blocks [0, nbM) are the metal pass (G lanes per atom: EAM pairs + LJ with
 //            carbon), blocks [nbM, nbM + nbC) the carbon pass (one thread per atom: bond, plane and
 //            angle terms from the close list + LJ with metal).  The epilogue writes acceleration,
@@ -32,15 +36,32 @@ __device__ __forceinline__ R disp(const P4 &a, const P4 &b, const DevParams &p, 
 
 constexpr int kChunk = 8;  // neighbours gathered per round
 
+// Eight consecutive list entries (two 16-byte loads; rows are 32-byte aligned); entries at or past
+// `valid` are replaced by `pad` so they index a real atom.
+__device__ __forceinline__ void load_chunk(const unsigned *row, int valid, unsigned pad, unsigned (&ent)[kChunk]) {
+    const uint4 a = *(const uint4 *)row, b = *(const uint4 *)(row + 4);
+    ent[0] = a.x;
+    ent[1] = a.y;
+    ent[2] = a.z;
+    ent[3] = a.w;
+    ent[4] = b.x;
+    ent[5] = b.y;
+    ent[6] = b.z;
+    ent[7] = b.w;
+#pragma unroll
+    for (int t = 0; t < kChunk; ++t)
+        if (t >= valid) ent[t] = pad;
+}
+
 // EamComputeRhoK (SurfaceGrowth.cu:716-773) with G lanes per atom; pair math in T.
 template <bool M, class T, int G>
-__global__ __launch_bounds__(256) void k_rho(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                             const int *__restrict__ nlM, const int *__restrict__ nnM,
-                                             float *__restrict__ rho, float *__restrict__ embF,
-                                             float *__restrict__ embDF) {
+__device__ __forceinline__ void rho_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                         const int *__restrict__ nlM, const int *__restrict__ nnM,
+                                         float *__restrict__ rho, float *__restrict__ embF,
+                                         float *__restrict__ embDF) {
     using R = typename Prec<M>::R;
     using P4 = typename Prec<M>::P4;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / G, l = gt % G;
     const bool active = i < p.nMolMe;
     T sum = T(0);
@@ -74,6 +95,58 @@ __global__ __launch_bounds__(256) void k_rho(DevParams p, const typename Prec<M>
         embF[i] = (float)eam_embed<T>(p, sum, dF);
         embDF[i] = (float)dF;
     }
+}
+
+// Near lists of one carbon from its close list at the current positions (the bond search of
+// SurfaceGrowth.cu:917-931 and the neighbour search of 940-951, cutoff tests in storage precision):
+//   bondC = {first three carbons with r^2 < rrCutC, as k<<8 | list position; count}
+//   bondA = {first three atoms of any kind with r^2 < rrCutC; count}
+// The close list holds every neighbour-list entry within rCutC (the displacement bound keeps it
+// exact), in list order, so these are the entries the reference's loops find.
+template <bool M>
+__device__ __forceinline__ void bonds_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                           const unsigned *__restrict__ cl, const int *__restrict__ ncl,
+                                           uint4 *__restrict__ bondC, uint4 *__restrict__ bondA) {
+    using R = typename Prec<M>::R;
+    using P4 = typename Prec<M>::P4;
+    const int ic = blk * blockDim.x + threadIdx.x;
+    if (ic >= p.nC) return;
+    const int i = p.nMolMe + ic;
+    const P4 Ai = pos[i];
+    unsigned cb[3] = {0u, 0u, 0u}, ab[3] = {0u, 0u, 0u};
+    int nc = 0, na = 0;
+    const int nclose = ncl[ic];
+    for (int base = 0; base < nclose && (nc < 3 || na < 3); base += kChunk) {
+        unsigned ent[kChunk];
+        P4 B[kChunk];
+        load_chunk(cl + (size_t)ic * p.clCap + base, nclose - base, (unsigned)i << 8, ent);
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t) {
+            if (base + t >= nclose) continue;
+            R dx, dy, dz;
+            const R rr = disp<R>(Ai, B[t], p, dx, dy, dz);
+            if (!(rr < (R)p.rrCutC)) continue;
+            if (na < 3) ab[na++] = ent[t];
+            if (nc < 3 && (int)(ent[t] >> 8) >= p.nMolMe) cb[nc++] = ent[t];
+        }
+    }
+    bondC[ic] = make_uint4(cb[0], cb[1], cb[2], (unsigned)nc);
+    bondA[ic] = make_uint4(ab[0], ab[1], ab[2], (unsigned)na);
+}
+
+template <bool M, class T, int G>
+__global__ __launch_bounds__(256) void k_rho(DevParams p, int nbRho, const typename Prec<M>::P4 *__restrict__ pos,
+                                             const int *__restrict__ nlM, const int *__restrict__ nnM,
+                                             float *__restrict__ rho, float *__restrict__ embF,
+                                             float *__restrict__ embDF, const unsigned *__restrict__ cl,
+                                             const int *__restrict__ ncl, uint4 *__restrict__ bondC,
+                                             uint4 *__restrict__ bondA) {
+    if ((int)blockIdx.x < nbRho)
+        rho_body<M, T, G>(p, blockIdx.x, pos, nlM, nnM, rho, embF, embDF);
+    else
+        bonds_body<M>(p, blockIdx.x - nbRho, pos, cl, ncl, bondC, bondA);
 }
 
 // ---- metal pass -----------------------------------------------------------------------------
@@ -187,10 +260,12 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
 }
 
 // ---- carbon pass ----------------------------------------------------------------------------
-// Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form: in FP64
-// through sin(theta - 2pi/3) and cos(theta - 2pi/3), which has no cancellation near the
-// equilibrium angle, plus the constant (2pi/3 - theta_0) because theta_0 is the float 2pi/3 of the
-// reference.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
+// Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form:
+// sin(theta - 2pi/3) and cos(theta - 2pi/3) are formed in FP64 (the sine cancels near the
+// equilibrium angle, so it needs the extra precision), the arc from them in FP32 - the deviation is
+// an FP32 quantity of the force law, and atan2f of two correctly signed values carries a relative
+// error of a few ulp - plus the constant (2pi/3 - theta_0) because theta_0 is the float 2pi/3 of
+// the reference.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
 template <class R>
 struct AngleDev;
 template <>
@@ -208,7 +283,7 @@ struct AngleDev<double> {
         const double u = -0.5 * sd - h * c;        // sin(theta - 2pi/3)
         const double v = -0.5 * c + h * sd;        // cos(theta - 2pi/3)
         s = (float)sd;
-        return (float)(atan2(u, v) + (2.0943951023931954923 - (double)p.theta_0));
+        return (float)((double)atan2f((float)u, (float)v) + (2.0943951023931954923 - (double)p.theta_0));
     }
 };
 
@@ -244,7 +319,7 @@ template <bool M, class T>
 __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
     float *__restrict__ acc, const unsigned *__restrict__ lj, const int *__restrict__ nlj,
-    const unsigned *__restrict__ cl, const int *__restrict__ ncl, int integrate) {
+    const uint4 *__restrict__ bondC, const uint4 *__restrict__ bondA, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
     using P4 = typename Prec<M>::P4;
@@ -261,57 +336,41 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     float dZ[3];
     int kb[3];
     P4 Bk[3];
-    int counti = 0, j0 = -1;
     // Bonds: the first three carbon neighbours, in list order, closer than rCutC (.cu:921).
-    const int nclose = ncl[ic];
-    for (int base = 0; base < nclose && counti < 3; base += kChunk) {
-        unsigned ent[kChunk];
-        P4 B[kChunk];
+    const uint4 nbi = bondC[ic];
+    const int counti = (int)nbi.w;
+    const unsigned be[3] = {nbi.x, nbi.y, nbi.z};
+    const int j0 = counti == 3 ? (int)(nbi.z & 255u) : -1;
+    uint4 nbk[3];
 #pragma unroll
-        for (int t = 0; t < kChunk; ++t)
-            ent[t] = base + t < nclose ? cl[(size_t)(base + t) * p.nC + ic] : ((unsigned)i << 8);
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t) {
-            const int k = (int)(ent[t] >> 8);
-            if (base + t >= nclose || counti >= 3 || k < p.nMolMe) continue;
-            V3<R> rik;
-            const R rrr = disp<R>(Ai, B[t], p, rik.x, rik.y, rik.z);
-            if (!(rrr < (R)p.rrCutC)) continue;
-            rnn[counti] = rik;
-            absR[counti] = sqrt(rrr);
-            dZ[counti] = (float)(B[t].z - z0);
-            kb[counti] = k;
-            Bk[counti] = B[t];
-            ++counti;
-            if (counti == 3) j0 = (int)(ent[t] & 255u);
-        }
+    for (int b = 0; b < 3; ++b) {
+        kb[b] = b < counti ? (int)(be[b] >> 8) : i;
+        Bk[b] = pos[kb[b]];
+        nbk[b] = bondA[b < counti ? kb[b] - p.nMolMe : ic];
     }
-    // For each bond k: k's own bonded neighbours m != i, the first two in k's list order.
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        if (b >= counti) continue;
+        const R rrr = disp<R>(Ai, Bk[b], p, rnn[b].x, rnn[b].y, rnn[b].z);
+        absR[b] = sqrt(rrr);
+        dZ[b] = (float)(Bk[b].z - z0);
+    }
+    // For each bond k: k's own near neighbours m != i, the first two in k's list order (.cu:940-978).
     for (int b = 0; b < counti; ++b) {
-        const int kc = kb[b] - p.nMolMe;
-        const int nk = ncl[kc];
+        const unsigned me[3] = {nbk[b].x, nbk[b].y, nbk[b].z};
+        const int na = (int)nbk[b].w;
+        P4 Bm[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) Bm[t] = pos[t < na ? (int)(me[t] >> 8) : i];
         int countk = 0;
-        for (int base = 0; base < nk && countk < 2; base += kChunk) {
-            unsigned ent[kChunk];
-            P4 B[kChunk];
 #pragma unroll
-            for (int t = 0; t < kChunk; ++t)
-                ent[t] = base + t < nk ? cl[(size_t)(base + t) * p.nC + kc] : ((unsigned)i << 8);
-#pragma unroll
-            for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
-#pragma unroll
-            for (int t = 0; t < kChunk; ++t) {
-                const int m = (int)(ent[t] >> 8);
-                if (base + t >= nk || countk >= 2 || m == i) continue;
-                V3<R> e;
-                const R rr2 = disp<R>(Bk[b], B[t], p, e.x, e.y, e.z);
-                if (!(rr2 < (R)p.rrCutC)) continue;
-                const V3<R> em = {-e.x, -e.y, -e.z};
-                neighbour_angle<R>(p, rnn[b], absR[b], em, rr2, (float)(B[t].z - z0), fx, fy, fz);
-                ++countk;
-            }
+        for (int t = 0; t < 3; ++t) {
+            if (t >= na || countk >= 2 || (int)(me[t] >> 8) == i) continue;
+            V3<R> e;
+            const R rr2 = disp<R>(Bk[b], Bm[t], p, e.x, e.y, e.z);
+            const V3<R> em = {-e.x, -e.y, -e.z};
+            neighbour_angle<R>(p, rnn[b], absR[b], em, rr2, (float)(Bm[t].z - z0), fx, fy, fz);
+            ++countk;
         }
     }
     if (counti == 3) {  // SurfaceGrowth.cu:985-1061
@@ -405,15 +464,15 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
                                                const int *__restrict__ nlM, const int *__restrict__ nnM,
                                                const float *__restrict__ embF, const float *__restrict__ embDF,
                                                float *__restrict__ fric, const unsigned *__restrict__ lj,
-                                               const int *__restrict__ nlj, const unsigned *__restrict__ cl,
-                                               const int *__restrict__ ncl, Partial<typename Prec<M>::A> *part,
+                                               const int *__restrict__ nlj, const uint4 *__restrict__ bondC,
+                                               const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
                                                int integrate, DevState *st, sgk_record *records, int recCap) {
     using A = typename Prec<M>::A;
     Partial<A> q;
     if ((int)blockIdx.x < nbM)
         q = metal_body<M, T, G>(p, blockIdx.x, pos, vel, acc, nlM, nnM, embF, embDF, fric, integrate);
     else
-        q = carbon_body<M, T>(p, blockIdx.x - nbM, pos, vel, acc, lj, nlj, cl, ncl, integrate);
+        q = carbon_body<M, T>(p, blockIdx.x - nbM, pos, vel, acc, lj, nlj, bondC, bondA, integrate);
     const Partial<A> r = block_reduce_partial(q);
     __shared__ int last;
     if (threadIdx.x == 0) {
@@ -438,11 +497,11 @@ template <bool M, class T>
 static void rho_dispatch(sgk_ctx *c) {
     using P4 = typename Prec<M>::P4;
     const int G = c->metalLanes;
-    const unsigned g = blocks_for((long long)c->dp.nMolMe * G, 256);
+    const unsigned g = (unsigned)(c->nbRho + c->nbC);
     const P4 *pos = (const P4 *)c->pos;
 #define SGK_RHO(GG)                                                                                              \
-    hipLaunchKernelGGL((k_rho<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, pos, c->nlM, c->nnM, c->rho, \
-                       c->embF, c->embDF)
+    hipLaunchKernelGGL((k_rho<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbRho, pos, c->nlM, c->nnM, \
+                       c->rho, c->embF, c->embDF, c->cl, c->ncl, c->bondC, c->bondA)
     switch (G) {
         case 1: SGK_RHO(1); break;
         case 2: SGK_RHO(2); break;
@@ -466,7 +525,7 @@ static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
     DevState *st = finalize ? c->state : nullptr;
 #define SGK_F(GG)                                                                                                \
     hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, c->acc,    \
-                       c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->cl, c->ncl, part,           \
+                       c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->bondC, c->bondA, part,     \
                        (int)integrate, st, c->records, c->cfg.records_capacity)
     switch (G) {
         case 1: SGK_F(1); break;
@@ -479,7 +538,7 @@ static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
 }
 
 void launch_rho(sgk_ctx *c) {
-    if (c->dp.nMolMe == 0) return;
+    if (c->nbRho + c->nbC == 0) return;
     profile_begin(c);
     if (!c->mixed)
         rho_dispatch<false, float>(c);

@@ -46,6 +46,7 @@ struct sgk_ctx {
     int *nlM = nullptr, *nnM = nullptr;                 // metal: row-major [nMolMe*nlCap]
     unsigned int *lj = nullptr, *cl = nullptr;          // carbon: column-major packed (idx<<8|pos)
     int *nlj = nullptr, *ncl = nullptr;
+    uint4 *bondC = nullptr, *bondA = nullptr;          // carbon: per-step near lists (k_rho)
     int *fullNN = nullptr, *fullNBL = nullptr;          // reference layout (keep_full_nl)
     // reductions and run state
     void *partials = nullptr;
@@ -54,7 +55,7 @@ struct sgk_ctx {
     unsigned int *hist = nullptr;
     sgk::DevState *hostState = nullptr;                 // pinned mirror
     // launch geometry
-    int block = 256, metalLanes = 8, nbM = 0, nbC = 0, nbProps = 0;
+    int block = 256, metalLanes = 8, nbM = 0, nbC = 0, nbRho = 0, nbProps = 0;
     long long bytes = 0;
     bool haveList = false;
     int hostStep = 0;                                   // host copy of the step counter

@@ -4,13 +4,13 @@
 //   binning     cell index per atom with the reference formula, slot by atomicAdd into nxt; the
 //               first atom of a cell appends it to nxt's non-empty list (bin_atom; in the step loop
 //               it runs inside the leapfrog kernel).
-//   k_nl_cell   one wave per non-empty cell of nxt: the 27 neighbour cells are staged in LDS as
-//               tiles sorted ascending in registers (bitonic), so every atom's list comes out in the
-//               reference order; the step path also zeroes cur's counts for the next rebuild.
+//   k_sortcells one wave per non-empty cell of nxt sorts its slots ascending (bitonic, in
+//               registers), so traversals see atoms in index order; zeroes cur's counts for the
+//               rebuild after this one.
+//   k_nl        one thread per atom: the reference neighbour order from the sorted cells.
 //   commit      cellbuf = nxt; cur's non-empty count = 0 (the step reduction, or k_commit).
-// The API path (sgk_bin) bins with k_bin and sorts the stored cells with k_sortcells.  In the
-// device-resident loop the kernels return at once unless state.nebrNow is set, so a captured step
-// needs no host decision.
+// The API path (sgk_bin) bins with k_bin.  In the device-resident loop the kernels return at once
+// unless state.nebrNow is set, so a captured step needs no host decision.
 //
 // Cell index, neighbour-list membership and order, and RDF bins are integer results defined by the
 // reference's FP32 arithmetic; both precision forms evaluate them in FP32 on the float-rounded
@@ -114,118 +114,107 @@ __global__ void k_sortcells(DevParams p, CellBufs bufs, DevState *st, int force)
     for (int e = tid; e < neo; e += nth) bufs.cnt[old][bufs.nonempty[old][e]] = 0;
 }
 
-// Bitonic sort of one value per lane across the wave (64 lanes), ascending.
-__device__ __forceinline__ int wave_sort_asc(int v, int lane) {
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int w = __shfl_xor(v, j, 64);
-            const bool lower = (lane & j) == 0, up = (lane & k) == 0;
-            v = (lower == up) ? min(v, w) : max(v, w);
-        }
-    }
-    return v;
-}
-
-// BuildNebrListK (SurfaceGrowth.cu:612-711), Anderson et al. algorithm 4 as in the reference: one
-// wave per home cell, lane l = l-th atom of the cell; for each of the 27 offset cells in table
-// order the cell is loaded once (one atom per lane), sorted, staged in LDS, and every home atom
-// tests the tile in ascending order.  Outputs:
+// BuildNebrListK (SurfaceGrowth.cu:612-711), one thread per atom over cells whose slots are sorted
+// ascending (k_sortcells), neighbours appended in the reference order: 27 offset cells in table
+// order, atoms of a cell ascending.  Slots are read 8 at a time with vector loads and their
+// positions gathered together; the next cell's count is prefetched.  Outputs:
 //   metal atom:  nlM[i*nlCap + j] = k                     (all neighbours)
 //   carbon atom: lj[q*nC + ic] = k<<8 | j   for metal k   (LJ partners with list position j)
-//                cl[q*nC + ic] = k<<8 | j   if r^2 < closeR2 (bond and angle candidates)
+//                cl[ic*clCap + q] = k<<8 | j if r^2 < closeR2 (bond and angle candidates)
 //   keepFull:    fullNBL[j*nMol + i] = k, fullNN[i]      (reference layout)
 template <bool M>
-__global__ __launch_bounds__(64) void k_nl_cell(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                                CellBufs bufs, DevState *st, int force, int fromNext, int zeroOld,
-                                                int *__restrict__ nlM, int *__restrict__ nnM,
-                                                unsigned *__restrict__ lj, int *__restrict__ nlj,
-                                                unsigned *__restrict__ cl, int *__restrict__ ncl,
-                                                int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+__global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                            const int *__restrict__ cellOf, CellBufs bufs, DevState *st, int force,
+                                            int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
+                                            unsigned *__restrict__ lj, int *__restrict__ nlj,
+                                            unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                            int *__restrict__ fullNN, int *__restrict__ fullNBL) {
     if (!force && !st->nebrNow) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nMol) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
     const int *cnt = bufs.cnt[b];
     const int *slots = bufs.slots[b];
-    const int lane = threadIdx.x;
-    __shared__ int tk[64];
-    __shared__ float tx[64], ty[64], tz[64];
-    const int ne = st->nonempty[b];
-    const float rrN = p.rrNebr, rrClose = p.closeR2;
-    for (int e = blockIdx.x; e < ne; e += gridDim.x) {
-        const int c = bufs.nonempty[b][e];
-        const int nh = min(cnt[c], p.cellCap);
-        const int i = lane < nh ? slots[(size_t)c * p.cellCap + lane] : -1;
-        const float3 A = i >= 0 ? f3(pos[i]) : make_float3(0.f, 0.f, 0.f);
+    const bool metal = i < p.nMolMe;
+    const int ic = i - p.nMolMe;
+    const int c = cellOf[i];
+    int nn = 0, nljv = 0, nclv = 0, err = 0;
+    if (c >= 0) {
         const int ccz = c / (p.cx * p.cy);
         const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
         const int ccx = c - (ccz * p.cy + ccy) * p.cx;
-        const bool metal = i >= 0 && i < p.nMolMe;
-        const int ic = i - p.nMolMe;
-        int nn = 0, nljv = 0, nclv = 0, err = 0;
+        const float3 A = f3(pos[i]);
+        const float rrN = p.rrNebr, rrClose = p.closeR2;
+        float shx, shy, shz;
+        int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
+        int ncur = cnt[C];
         for (int o = 0; o < 27; ++o) {
-            float shx, shy, shz;
-            const int C = stencil_cell<float>(p, ccx, ccy, ccz, o, shx, shy, shz);
-            const int nt = min(cnt[C], p.cellCap);
-            int v = lane < nt ? slots[(size_t)C * p.cellCap + lane] : 0x7fffffff;
-            v = wave_sort_asc(v, lane);
-            const float3 B = lane < nt ? f3(pos[v]) : make_float3(0.f, 0.f, 0.f);
-            __syncthreads();
-            tk[lane] = v;
-            tx[lane] = B.x;
-            ty[lane] = B.y;
-            tz[lane] = B.z;
-            __syncthreads();
-            if (i < 0) continue;
-            for (int t = 0; t < nt; ++t) {
-                const int k = tk[t];
-                const float rr = shifted_rr(A, make_float3(tx[t], ty[t], tz[t]), shx, shy, shz);
-                if (!(rr < rrN && k != i)) continue;
-                if (nn < p.nlCap) {
-                    if (p.keepFull) fullNBL[(size_t)nn * p.nMol + i] = k;
-                    if (metal) {
-                        nlM[(size_t)i * p.nlCap + nn] = k;
-                    } else {
-                        const unsigned ent = ((unsigned)k << 8) | (unsigned)nn;
-                        if (k < p.nMolMe) {
-                            if (nljv < p.ljCap)
-                                lj[(size_t)nljv * p.nC + ic] = ent;
-                            else
-                                err |= SGK_FLAG_LJ_OVERFLOW;
-                            ++nljv;
-                        }
-                        if (rr < rrClose) {
-                            if (nclv < p.clCap)
-                                cl[(size_t)nclv * p.nC + ic] = ent;
-                            else
-                                err |= SGK_FLAG_CLOSE_OVERFLOW;
-                            ++nclv;
-                        }
-                    }
-                } else {
-                    err |= SGK_FLAG_NL_OVERFLOW;
+            float nshx = 0.f, nshy = 0.f, nshz = 0.f;
+            int Cn = C, nnext = 0;
+            if (o < 26) {  // prefetch the next cell's count
+                Cn = stencil_cell<float>(p, ccx, ccy, ccz, o + 1, nshx, nshy, nshz);
+                nnext = cnt[Cn];
+            }
+            const int n = min(ncur, p.cellCap);
+            const int *s = slots + (size_t)C * p.cellCap;
+            for (int base = 0; base < n; base += 8) {
+                const int4 k0 = *(const int4 *)(s + base);
+                const int4 k1 = *(const int4 *)(s + base + 4);
+                int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+                float3 B[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    if (base + t >= n) k[t] = i;  // slots past the count are not defined
+                    B[t] = f3(pos[k[t]]);
                 }
-                ++nn;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    if (base + t >= n) continue;
+                    const float rr = shifted_rr(A, B[t], shx, shy, shz);
+                    if (!(rr < rrN && k[t] != i)) continue;
+                    if (nn < p.nlCap) {
+                        if (p.keepFull) fullNBL[(size_t)nn * p.nMol + i] = k[t];
+                        if (metal) {
+                            nlM[(size_t)i * p.nlCap + nn] = k[t];
+                        } else {
+                            const unsigned ent = ((unsigned)k[t] << 8) | (unsigned)nn;
+                            if (k[t] < p.nMolMe) {
+                                if (nljv < p.ljCap)
+                                    lj[(size_t)nljv * p.nC + ic] = ent;
+                                else
+                                    err |= SGK_FLAG_LJ_OVERFLOW;
+                                ++nljv;
+                            }
+                            if (rr < rrClose) {
+                                if (nclv < p.clCap)
+                                    cl[(size_t)ic * p.clCap + nclv] = ent;
+                                else
+                                    err |= SGK_FLAG_CLOSE_OVERFLOW;
+                                ++nclv;
+                            }
+                        }
+                    } else {
+                        err |= SGK_FLAG_NL_OVERFLOW;
+                    }
+                    ++nn;
+                }
             }
-        }
-        if (i >= 0) {
-            nn = min(nn, p.nlCap);
-            if (p.keepFull) fullNN[i] = nn;
-            if (metal) {
-                nnM[i] = nn;
-            } else {
-                nlj[ic] = min(nljv, p.ljCap);
-                ncl[ic] = min(nclv, p.clCap);
-            }
-            if (err) atomicOr(&st->errors, err);
+            C = Cn;
+            ncur = nnext;
+            shx = nshx;
+            shy = nshy;
+            shz = nshz;
         }
     }
-    if (zeroOld) {  // the outgoing buffer becomes the spare: clear its counts for the next rebuild
-        const int old = b ^ 1;
-        const int neo = st->nonempty[old];
-        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < neo; e += gridDim.x * blockDim.x)
-            bufs.cnt[old][bufs.nonempty[old][e]] = 0;
+    nn = min(nn, p.nlCap);
+    if (p.keepFull) fullNN[i] = nn;
+    if (metal) {
+        nnM[i] = nn;
+    } else {
+        nlj[ic] = min(nljv, p.ljCap);
+        ncl[ic] = min(nclv, p.clCap);
     }
+    if (err) atomicOr(&st->errors, err);
 }
 
 // Make the freshly built buffer current (API path; the step path does this in its reduction).
@@ -310,16 +299,16 @@ void launch_sortcells(sgk_ctx *c, bool force) {
 void launch_nl(sgk_ctx *c, bool force) {
     if (c->dp.nMol == 0) return;
     profile_begin(c);
-    const unsigned g = (unsigned)std::min(c->dp.ncell, 8192);
-    const int step = force ? 0 : 1;  // step path: build from the new buffer, clear the old one
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    const int fromNext = force ? 0 : 1;  // step path: build from the new buffer before the commit
     if (c->mixed)
-        hipLaunchKernelGGL(k_nl_cell<true>, dim3(g), dim3(64), 0, c->stream, c->dp, (const double4 *)c->pos,
-                           cell_bufs(c), c->state, (int)force, step, step, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
-                           c->ncl, c->fullNN, c->fullNBL);
+        hipLaunchKernelGGL(k_nl<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
+                           c->fullNN, c->fullNBL);
     else
-        hipLaunchKernelGGL(k_nl_cell<false>, dim3(g), dim3(64), 0, c->stream, c->dp, (const float4 *)c->pos,
-                           cell_bufs(c), c->state, (int)force, step, step, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
-                           c->ncl, c->fullNN, c->fullNBL);
+        hipLaunchKernelGGL(k_nl<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
+                           cell_bufs(c), c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
+                           c->fullNN, c->fullNBL);
     profile_end(c, "nl");
 }
 
