@@ -296,7 +296,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
     SGK_TRY(dalloc(c, (char **)&c->pos, nMol * (c->mixed ? 32 : 16), true));
     SGK_TRY(dalloc(c, (char **)&c->vel, 3 * nMol * R, true));
-    SGK_TRY(dalloc(c, &c->acc, 3 * nMol, true));
+    SGK_TRY(dalloc(c, (char **)&c->acc, 3 * nMol * R, true));
     SGK_TRY(dalloc(c, &c->rho, nMe, true));
     SGK_TRY(dalloc(c, &c->embF, nMe, true));
     SGK_TRY(dalloc(c, &c->embDF, nMe, true));
@@ -403,13 +403,23 @@ int sgk_upload(sgk_ctx *c, const sg_float4 *r, const sg_float3 *v, const sg_floa
         }
     }
     if (a) {
-        std::vector<float> t(3 * n);
-        for (size_t i = 0; i < n; ++i) {
-            t[i] = a[i].x;
-            t[n + i] = a[i].y;
-            t[2 * n + i] = a[i].z;
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            for (size_t i = 0; i < n; ++i) {
+                t[i] = a[i].x;
+                t[n + i] = a[i].y;
+                t[2 * n + i] = a[i].z;
+            }
+            SGK_HIP(c, hipMemcpy(c->acc, t.data(), 8 * 3 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> t(3 * n);
+            for (size_t i = 0; i < n; ++i) {
+                t[i] = a[i].x;
+                t[n + i] = a[i].y;
+                t[2 * n + i] = a[i].z;
+            }
+            SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
         }
-        SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
     }
     return SGK_OK;
 }
@@ -441,10 +451,17 @@ int sgk_upload_f64(sgk_ctx *c, const double *r, const double *v, const double *a
         }
     }
     if (a) {
-        std::vector<float> t(3 * n);
-        for (size_t i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) t[d * n + i] = (float)a[3 * i + d];
-        SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) t[d * n + i] = a[3 * i + d];
+            SGK_HIP(c, hipMemcpy(c->acc, t.data(), 8 * 3 * n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> t(3 * n);
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) t[d * n + i] = (float)a[3 * i + d];
+            SGK_HIP(c, hipMemcpy(c->acc, t.data(), 4 * 3 * n, hipMemcpyHostToDevice));
+        }
     }
     return SGK_OK;
 }
@@ -476,10 +493,17 @@ int sgk_download_f64(sgk_ctx *c, double *r, double *v, double *a) {
         }
     }
     if (a) {
-        std::vector<float> t(3 * n);
-        SGK_HIP(c, hipMemcpy(t.data(), c->acc, 4 * 3 * n, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) a[3 * i + d] = t[d * n + i];
+        if (c->mixed) {
+            std::vector<double> t(3 * n);
+            SGK_HIP(c, hipMemcpy(t.data(), c->acc, 8 * 3 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) a[3 * i + d] = t[d * n + i];
+        } else {
+            std::vector<float> t(3 * n);
+            SGK_HIP(c, hipMemcpy(t.data(), c->acc, 4 * 3 * n, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) a[3 * i + d] = t[d * n + i];
+        }
     }
     return SGK_OK;
 }

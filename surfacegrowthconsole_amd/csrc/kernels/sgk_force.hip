@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_rho(DevParams p, int nbRho, const typen
 template <bool M, class T, int G>
 __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
     const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
-    float *__restrict__ acc, const int *__restrict__ nlM, const int *__restrict__ nnM, const float *__restrict__ embF,
+    typename Prec<M>::R *__restrict__ acc, const int *__restrict__ nlM, const int *__restrict__ nnM, const float *__restrict__ embF,
     const float *__restrict__ embDF, float *__restrict__ fric, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
@@ -221,7 +221,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
     Partial<A> q;
     partial_identity(q);
     if (active && l == 0) {
-        const float ax = (float)(fx / (T)p.massMe), ay = (float)(fy / (T)p.massMe), az = (float)(fz / (T)p.massMe);
+        const R ax = (R)(fx / (T)p.massMe), ay = (R)(fy / (T)p.massMe), az = (R)(fz / (T)p.massMe);
         acc[i] = ax;
         acc[p.nMol + i] = ay;
         acc[2 * p.nMol + i] = az;
@@ -232,9 +232,9 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
         if (integrate) {
 #pragma clang fp contract(off)
             const R hdt = (R)(0.5f * p.deltaT);
-            vx = vx + hdt * (R)ax;
-            vy = vy + hdt * (R)ay;
-            vz = vz + hdt * (R)az;
+            vx = vx + hdt * ax;
+            vy = vy + hdt * ay;
+            vz = vz + hdt * az;
             vel[i] = vx;
             vel[p.nMol + i] = vy;
             vel[2 * p.nMol + i] = vz;
@@ -318,7 +318,7 @@ __device__ __forceinline__ void neighbour_angle(const DevParams &p, const V3<R> 
 template <bool M, class T>
 __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
-    float *__restrict__ acc, const unsigned *__restrict__ lj, const int *__restrict__ nlj,
+    typename Prec<M>::R *__restrict__ acc, const unsigned *__restrict__ lj, const int *__restrict__ nlj,
     const uint4 *__restrict__ bondC, const uint4 *__restrict__ bondA, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
@@ -427,12 +427,10 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
             gu += ul;
         }
     }
-    fx = (float)gx;
-    fy = (float)gy;
-    fz = (float)gz;
-    acc[i] = fx;
-    acc[p.nMol + i] = fy;
-    acc[2 * p.nMol + i] = fz;
+    const R ax = (R)gx, ay = (R)gy, az = (R)gz;  // carbon mass is the unit
+    acc[i] = ax;
+    acc[p.nMol + i] = ay;
+    acc[2 * p.nMol + i] = az;
     const float e = (float)(T(0.5f) * gu);
     pos[i].w = (R)e;
     R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
@@ -440,9 +438,9 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     if (integrate && move) {
 #pragma clang fp contract(off)
         const R hdt = (R)(0.5f * p.deltaT);
-        vx = vx + hdt * (R)fx;
-        vy = vy + hdt * (R)fy;
-        vz = vz + hdt * (R)fz;
+        vx = vx + hdt * ax;
+        vy = vy + hdt * ay;
+        vz = vz + hdt * az;
         vel[i] = vx;
         vel[p.nMol + i] = vy;
         vel[2 * p.nMol + i] = vz;
@@ -460,7 +458,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
 // One launch for both passes; the last block to finish reduces the partials of all blocks.
 template <bool M, class T, int G>
 __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
-                                               typename Prec<M>::R *__restrict__ vel, float *__restrict__ acc,
+                                               typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
                                                const int *__restrict__ nlM, const int *__restrict__ nnM,
                                                const float *__restrict__ embF, const float *__restrict__ embDF,
                                                float *__restrict__ fric, const unsigned *__restrict__ lj,
@@ -524,7 +522,7 @@ static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
     Partial<A> *part = (Partial<A> *)c->partials;
     DevState *st = finalize ? c->state : nullptr;
 #define SGK_F(GG)                                                                                                \
-    hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, c->acc,    \
+    hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, (R *)c->acc, \
                        c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->bondC, c->bondA, part,     \
                        (int)integrate, st, c->records, c->cfg.records_capacity)
     switch (G) {

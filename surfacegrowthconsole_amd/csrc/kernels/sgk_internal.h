@@ -32,10 +32,10 @@ struct sgk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
 
-    // state arrays (positions AoS x,y,z,E; velocity and acceleration SoA)
+    // state arrays (positions AoS x,y,z,E; velocity and acceleration SoA, storage precision R)
     void *pos = nullptr;   // float4 or double4 [nMol]
     void *vel = nullptr;   // R [3*nMol]
-    float *acc = nullptr;  // [3*nMol]
+    void *acc = nullptr;   // R [3*nMol] (FP64 in the mixed form)
     float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;  // [nMolMe]
     // cells (two buffers, see the rebuild protocol in sgk_nebr.hip)
     int *cellOf = nullptr;

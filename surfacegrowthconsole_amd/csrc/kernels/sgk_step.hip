@@ -17,7 +17,8 @@ namespace sgk {
 // (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
 template <bool M>
 __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
-                                             typename Prec<M>::R *__restrict__ vel, const float *__restrict__ acc,
+                                             typename Prec<M>::R *__restrict__ vel,
+                                             const typename Prec<M>::R *__restrict__ acc,
                                              int *__restrict__ cellOf, CellBufs bufs, DevState *st, int doBin) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
@@ -30,9 +31,9 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
     if (move) {
         const R hdt = (R)(0.5f * p.deltaT), dt = (R)p.deltaT;
         R vx = vel[n], vy = vel[p.nMol + n], vz = vel[2 * p.nMol + n];
-        vx = vx + hdt * (R)acc[n];
-        vy = vy + hdt * (R)acc[p.nMol + n];
-        vz = vz + hdt * (R)acc[2 * p.nMol + n];
+        vx = vx + hdt * acc[n];
+        vy = vy + hdt * acc[p.nMol + n];
+        vz = vz + hdt * acc[2 * p.nMol + n];
         vel[n] = vx;
         vel[p.nMol + n] = vy;
         vel[2 * p.nMol + n] = vz;
@@ -54,7 +55,8 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
 // LeapfrogStepK part 2 alone (the step loop fuses it into the force kernels).
 template <bool M>
 __global__ __launch_bounds__(256) void k_lf2(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                             typename Prec<M>::R *__restrict__ vel, const float *__restrict__ acc) {
+                                             typename Prec<M>::R *__restrict__ vel,
+                                             const typename Prec<M>::R *__restrict__ acc) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -62,9 +64,9 @@ __global__ __launch_bounds__(256) void k_lf2(DevParams p, const typename Prec<M>
     const auto r = pos[n];
     if (n >= p.nMolMe && !(r.x != (R)p.xLeft && r.x < (R)p.xRight && r.y != (R)p.yTop && r.y != (R)p.yBottom)) return;
     const R hdt = (R)(0.5f * p.deltaT);
-    vel[n] = vel[n] + hdt * (R)acc[n];
-    vel[p.nMol + n] = vel[p.nMol + n] + hdt * (R)acc[p.nMol + n];
-    vel[2 * p.nMol + n] = vel[2 * p.nMol + n] + hdt * (R)acc[2 * p.nMol + n];
+    vel[n] = vel[n] + hdt * acc[n];
+    vel[p.nMol + n] = vel[p.nMol + n] + hdt * acc[p.nMol + n];
+    vel[2 * p.nMol + n] = vel[2 * p.nMol + n] + hdt * acc[2 * p.nMol + n];
 }
 
 // Per-atom contributions of the current state (the reductions of SurfaceGrowth.cu:1101-1891).
@@ -146,14 +148,15 @@ __global__ __launch_bounds__(256) void k_thermostat(DevParams p, typename Prec<M
 // acceleration; the count goes into the step's record.
 template <bool M>
 __global__ __launch_bounds__(256) void k_shear(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                               float *__restrict__ acc, const DevState *st, sgk_record *records,
+                                               typename Prec<M>::R *__restrict__ acc, const DevState *st,
+                                               sgk_record *records,
                                                int recCap) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     int hit = 0;
     if (n < p.nMolMe) {
         const float dr = (float)st->comX - (float)pos[n].x;
         if (dr > 0.f) {
-            acc[n] += (float)st->shear;
+            acc[n] += (typename Prec<M>::R)st->shear;
             hit = 1;
         }
     }
@@ -174,10 +177,10 @@ void launch_lf1(sgk_ctx *c, bool bin) {
     const unsigned g = blocks_for(c->dp.nMol, 256);
     if (c->mixed)
         hipLaunchKernelGGL(k_lf1<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double4 *)c->pos, (double *)c->vel,
-                           c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+                           (const double *)c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
     else
         hipLaunchKernelGGL(k_lf1<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos, (float *)c->vel,
-                           c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+                           (const float *)c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
     profile_end(c, "lf1");
 }
 
@@ -186,10 +189,10 @@ void launch_lf2(sgk_ctx *c) {
     const unsigned g = blocks_for(c->dp.nMol, 256);
     if (c->mixed)
         hipLaunchKernelGGL(k_lf2<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos,
-                           (double *)c->vel, c->acc);
+                           (double *)c->vel, (const double *)c->acc);
     else
         hipLaunchKernelGGL(k_lf2<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos,
-                           (float *)c->vel, c->acc);
+                           (float *)c->vel, (const float *)c->acc);
     profile_end(c, "lf2");
 }
 
@@ -237,11 +240,11 @@ void launch_shear(sgk_ctx *c) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMolMe, 256);
     if (c->mixed)
-        hipLaunchKernelGGL(k_shear<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->acc,
-                           c->state, c->records, c->cfg.records_capacity);
+        hipLaunchKernelGGL(k_shear<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos,
+                           (double *)c->acc, c->state, c->records, c->cfg.records_capacity);
     else
-        hipLaunchKernelGGL(k_shear<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->acc,
-                           c->state, c->records, c->cfg.records_capacity);
+        hipLaunchKernelGGL(k_shear<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos,
+                           (float *)c->acc, c->state, c->records, c->cfg.records_capacity);
     profile_end(c, "shear");
 }
 

@@ -153,8 +153,6 @@ def test_integrator_thermostat_props(gpu, prec):
     orc = oracle.Oracle(p, "f32" if prec == "fp32" else "f64")
     if prec == "fp32":
         r, v, a = r.astype(np.float32), v.astype(np.float32), a.astype(np.float32)
-    else:  # both forms store accelerations in FP32
-        a = a.astype(np.float32).astype(np.float64)
     with sg.Engine(p, precision=prec) as eng:
         eng.upload(r, v, a)
         eng.leapfrog(1)
