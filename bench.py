@@ -42,6 +42,34 @@ def build_if_needed():
         __graft_entry__.build()
 
 
+def calibrate(gib=1.0, reps=5):
+    """A plain elementwise kernel of known traffic (out = -in, float32): reads and writes `gib` GiB per
+    launch.  Profiled with FETCH_SIZE / WRITE_SIZE it gives the factor that converts those counters
+    to bytes on this machine."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.neg float32", "bytes_read_per_launch": 4 * n,
+                      "bytes_written_per_launch": 4 * n, "launches": reps}))
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_force launch from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_force_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def b_alg(precision, f_me):
     return (96.0 if precision == "mixed" else 48.0) + 20.0 * f_me
 
@@ -147,7 +175,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calibrate", action="store_true", help="run the copy kernel for counter calibration")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -194,7 +226,7 @@ def main():
         "fp32": {"value": p.nMol * args.steps / res["fp32"]["wall"],
                  "ms_per_step": res["fp32"]["wall"] / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config),
                      "kernel": "k_force (fused metal+carbon force pass)", "us_per_launch": force_us,
                      "alg_bytes_per_atom_step": b_alg("mixed", f_me)},
         "kernels": kernels,

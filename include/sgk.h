@@ -60,6 +60,9 @@ typedef struct sgk_config {
     int metal_thermostat_all; /* thermostat metal after stepEquil too (reference: only while
                                  stepEquil < step < stepEquil + stepCool) */
     int pair_fp64;        /* mixed form only: EAM and LJ pair arithmetic in FP64 (default) or FP32 */
+    int xcd_map;          /* launch shape: bit 0 neighbour list, bit 1 density/near lists, bit 2 force
+                             pass - give each of the 8 XCDs a contiguous 1/8 of the atoms */
+    int bin_group;        /* binning: one atomicAdd per distinct cell of a wave instead of per atom */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with

@@ -103,7 +103,7 @@ class SgkConfig(C.Structure):
         ("precision", _i), ("nl_capacity", _i), ("cell_capacity", _i), ("lj_capacity", _i),
         ("close_capacity", _i), ("close_margin", _f), ("metal_lanes", _i), ("block_size", _i),
         ("keep_full_nl", _i), ("rebuild_every", _i), ("records_capacity", _i), ("use_graphs", _i),
-        ("metal_thermostat_all", _i), ("pair_fp64", _i),
+        ("metal_thermostat_all", _i), ("pair_fp64", _i), ("xcd_map", _i), ("bin_group", _i),
     ]
 
 

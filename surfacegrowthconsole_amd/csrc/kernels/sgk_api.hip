@@ -204,6 +204,8 @@ static void fill_dev_params(sgk_ctx *c) {
     p.clCap = c->cfg.close_capacity;
     p.keepFull = c->cfg.keep_full_nl;
     p.rebuildEvery = c->cfg.rebuild_every;
+    p.xcdMap = c->cfg.xcd_map;
+    p.binGroup = c->cfg.bin_group;
 }
 
 extern "C" {
@@ -219,6 +221,8 @@ void sgk_default_config(sgk_config *cfg) {
     cfg->records_capacity = 1024;
     cfg->use_graphs = 1;
     cfg->pair_fp64 = 1;  // mixed form: FP64 pair arithmetic (measured cost nil, error < 1e-5)
+    cfg->xcd_map = 6;    // density and force passes
+    cfg->bin_group = 1;
 }
 
 int sgk_device_count(void) {
@@ -285,12 +289,14 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const DevParams &p = c->dp;
     // lanes per metal atom: enough waves to cover the machine for small clusters
     int G = cfg.metal_lanes;
-    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe < 32768 ? 16 : 8;
+    // measured on MI355X: 2 lanes for 100k metal atoms, 4 for 4k, more for small clusters
+    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 2 : p.nMolMe >= 2048 ? 4 : 16;
     c->metalLanes = G;
     c->block = 256;
-    c->nbM = (int)(((long long)p.nMolMe * G + 255) / 256);
+    // metal ranges padded to a multiple of 8 workgroups so the carbon range starts on XCD 0
+    c->nbM = (int)((((long long)p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
     c->nbC = (p.nC + 255) / 256;
-    c->nbRho = (int)(((long long)p.nMolMe * G + 255) / 256);
+    c->nbRho = c->nbM;
     c->nbProps = (p.nMol + 255) / 256;
     const size_t R = c->mixed ? 8 : 4;
     const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
@@ -301,12 +307,17 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     SGK_TRY(dalloc(c, &c->embF, nMe, true));
     SGK_TRY(dalloc(c, &c->embDF, nMe, true));
     SGK_TRY(dalloc(c, &c->fric, nMe, true));
-    SGK_TRY(dalloc(c, &c->cellOf, nMol, true));
     for (int b = 0; b < 2; ++b) {
+        SGK_TRY(dalloc(c, &c->cellOf[b], nMol, false));
         SGK_TRY(dalloc(c, &c->cnt[b], (size_t)p.ncell, true));
         SGK_TRY(dalloc(c, &c->slots[b], (size_t)p.ncell * p.cellCap, false));
-        SGK_TRY(dalloc(c, &c->nonempty[b], (size_t)p.ncell, false));
     }
+    SGK_TRY(dalloc(c, &c->slotOf, nMol, false));
+    SGK_TRY(dalloc(c, &c->cellPos, (size_t)p.ncell * p.cellCap, false));
+    for (int b = 0; b < 2; ++b)  // -1: no cell yet (k_sortcells clears the outgoing buffer through it)
+        SGK_TRY(hipMemsetAsync(c->cellOf[b], 0xff, 4 * nMol, c->stream) == hipSuccess
+                    ? SGK_OK
+                    : fail(c, SGK_E_HIP, "hipMemsetAsync of the cell index"));
     SGK_TRY(dalloc(c, &c->nlM, nMe * (size_t)p.nlCap, false));
     SGK_TRY(dalloc(c, &c->nnM, nMe, true));
     SGK_TRY(dalloc(c, &c->lj, nC * (size_t)p.ljCap, false));
@@ -351,8 +362,8 @@ int sgk_destroy(sgk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-    void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf, c->cnt[0], c->cnt[1],
-                    c->slots[0], c->slots[1], c->nonempty[0], c->nonempty[1], c->nlM, c->nnM, c->lj, c->nlj, c->cl,
+    void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
+                    c->cnt[0], c->cnt[1], c->slots[0], c->slots[1], c->slotOf, c->cellPos, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
                     c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -382,6 +393,7 @@ int sgk_upload(sgk_ctx *c, const sg_float4 *r, const sg_float3 *v, const sg_floa
             SGK_HIP(c, hipMemcpy(c->pos, r, 16 * n, hipMemcpyHostToDevice));
         }
         c->haveList = false;
+        c->haveCells = false;
     }
     if (v) {
         if (c->mixed) {
@@ -436,6 +448,7 @@ int sgk_upload_f64(sgk_ctx *c, const double *r, const double *v, const double *a
             SGK_HIP(c, hipMemcpy(c->pos, t.data(), 16 * n, hipMemcpyHostToDevice));
         }
         c->haveList = false;
+        c->haveCells = false;
     }
     if (v) {
         if (c->mixed) {
@@ -541,6 +554,7 @@ int sgk_bin(sgk_ctx *c) {
     sgk::launch_sortcells(c, true);
     sgk::launch_commit_rebuild(c);
     SGK_HIP(c, hipGetLastError());
+    c->haveCells = true;
     return check_flags(c);
 }
 
@@ -549,7 +563,7 @@ int sgk_get_cells(sgk_ctx *c, int *cellOf, int *cellCount, int *cellSlots) {
     SGK_HIP(c, hipStreamSynchronize(c->stream));
     SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
     const int b = c->hostState->cellbuf;
-    if (cellOf) SGK_HIP(c, hipMemcpy(cellOf, c->cellOf, 4 * (size_t)c->dp.nMol, hipMemcpyDeviceToHost));
+    if (cellOf) SGK_HIP(c, hipMemcpy(cellOf, c->cellOf[b], 4 * (size_t)c->dp.nMol, hipMemcpyDeviceToHost));
     if (cellCount) SGK_HIP(c, hipMemcpy(cellCount, c->cnt[b], 4 * (size_t)c->dp.ncell, hipMemcpyDeviceToHost));
     if (cellSlots) {
         std::vector<int> cnt(c->dp.ncell);
@@ -563,6 +577,7 @@ int sgk_get_cells(sgk_ctx *c, int *cellOf, int *cellCount, int *cellSlots) {
 
 int sgk_build_nebr(sgk_ctx *c) {
     if (!c) return SGK_E_ARG;
+    if (!c->haveCells) return fail(c, SGK_E_STATE, "no cells for the current positions: call sgk_bin first");
     sgk::launch_nl(c, true);
     SGK_HIP(c, hipGetLastError());
     int rc = check_flags(c);
@@ -665,9 +680,10 @@ int sgk_get_forces(sgk_ctx *c, double *acc, double *energy, double *fric) {
 
 int sgk_leapfrog(sgk_ctx *c, int part) {
     if (!c) return SGK_E_ARG;
-    if (part == 1)
+    if (part == 1) {
         sgk::launch_lf1(c, false);
-    else
+        c->haveCells = false;
+    } else
         sgk::launch_lf2(c);
     SGK_HIP(c, hipGetLastError());
     SGK_HIP(c, hipStreamSynchronize(c->stream));
@@ -796,6 +812,7 @@ int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
     if (c->profiling) flush_profile(c);
     c->hostStep += nsteps;
     c->haveList = true;
+    c->haveCells = false;  // cells are those of the last rebuild, not of the current positions
     return check_flags(c);
 }
 

@@ -8,10 +8,15 @@
 namespace sgk {
 
 // Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).
+// cellOf[b][n] is atom n's cell in buffer b (-1: not binned); slotOf[n] its slot in the buffer
+// being built; cellPos[c*cap + s] the float position of the atom in slot s of cell c after the sort
+// (built with the buffer, read by the neighbour-list pass).
 struct CellBufs {
     int *cnt[2];
     int *slots[2];
-    int *nonempty[2];
+    int *cellOf[2];
+    int *slotOf;
+    float4 *cellPos;
 };
 
 inline CellBufs cell_bufs(sgk_ctx *c) {
@@ -19,42 +24,54 @@ inline CellBufs cell_bufs(sgk_ctx *c) {
     for (int k = 0; k < 2; ++k) {
         b.cnt[k] = c->cnt[k];
         b.slots[k] = c->slots[k];
-        b.nonempty[k] = c->nonempty[k];
+        b.cellOf[k] = c->cellOf[k];
     }
+    b.slotOf = c->slotOf;
+    b.cellPos = c->cellPos;
     return b;
 }
 
 // Cell of atom n (position q, float) with the reference's FP32 formula
 // c = VLinear(int((r + L/2) * invWidth)) (SurfaceGrowth.cu:594-600), truncation toward zero, and
-// its slot in the new buffer (cellbuf ^ 1) by atomicAdd; the first atom of a cell appends the cell
-// to the buffer's non-empty list.  Slot order is arbitrary; readers sort.
-__device__ __forceinline__ void bin_atom(const DevParams &p, int n, float3 q, int *__restrict__ cellOf,
-                                         const CellBufs &bufs, DevState *st) {
+// its slot in the new buffer (cellbuf ^ 1) by atomicAdd.  Slot order is arbitrary; k_sortcells
+// sorts.  Called by every active lane of a wave together (wave-collective).
+__device__ __forceinline__ void bin_atom(const DevParams &p, int n, float3 q, const CellBufs &bufs, DevState *st) {
 #pragma clang fp contract(off)
     const float sx = q.x + 0.5f * p.Lx;
     const float sy = q.y + 0.5f * p.Ly;
     const float sz = q.z + 0.5f * p.Lz;
     const int cx = (int)(sx * p.iwx), cy = (int)(sy * p.iwy), cz = (int)(sz * p.iwz);
     const int c = (cz * p.cy + cy) * p.cx + cx;
+    const int b = st->cellbuf ^ 1;
     if (c < 0 || c >= p.ncell) {  // no reference block has this index: the atom is not binned
-        cellOf[n] = -1;
+        bufs.cellOf[b][n] = -1;
         atomicOr(&st->errors, SGK_FLAG_UNBINNED);
         return;
     }
-    cellOf[n] = c;
-    const int b = st->cellbuf ^ 1;
-    const int slot = atomicAdd(&bufs.cnt[b][c], 1);
-    // Append newly occupied cells with one atomic per wave (a single counter hit once per cell
-    // serialises at the L2).
-    const unsigned long long first = __ballot(slot == 0);
-    if (first) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((long long)first) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&st->nonempty[b], __popcll(first));
-        base = __shfl(base, leader, 64);
-        if (slot == 0) bufs.nonempty[b][base + __popcll(first & ((1ull << lane) - 1ull))] = c;
+    bufs.cellOf[b][n] = c;
+    // Lanes of a wave that fall in the same cell (consecutive atoms are spatial neighbours) take
+    // their slots with one atomic: group the active lanes by cell, one pass per distinct cell,
+    // then one atomicAdd per group leader.  Device-scope atomics on one address serialise, so
+    // their number, not bandwidth, bounds binning.
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = p.binGroup ? __ballot(1) : 0ull, grp = 1ull << lane;
+    int leader = lane;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const int cl = __shfl(c, l, 64);
+        const unsigned long long m = __ballot(c == cl);
+        if (c == cl) {
+            grp = m;
+            leader = l;
+        }
+        todo &= ~m;
     }
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&bufs.cnt[b][c], __popcll(grp));
+    base = __shfl(base, leader, 64);
+    const int slot = base + __popcll(grp & below);
+    bufs.slotOf[n] = slot;
     if (slot < p.cellCap)
         bufs.slots[b][(size_t)c * p.cellCap + slot] = n;
     else

@@ -46,6 +46,18 @@ __device__ __forceinline__ float3 min_image(const P4 &a, const P4 &b, R Lx, R Ly
                        (float)wrap1<R>(a.z - b.z, Lz));
 }
 
+// gfx950 dispatches workgroups to its 8 XCDs round robin: workgroup b runs on XCD b % 8, and each
+// XCD has its own L2.  xcd_block maps the hardware index b of a range of n workgroups to a logical
+// index so that every XCD works on one contiguous 1/8 of the range.  Atoms with nearby indices are
+// spatial neighbours, so an XCD's gathers then stay inside the positions its own L2 holds.
+// The map is a bijection of [0, n); a range that starts at a hardware index that is a multiple of
+// 8 keeps the XCD of every workgroup.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int q = n / kXcds, r = n % kXcds, x = b % kXcds, k = b / kXcds;
+    return x * q + min(x, r) + k;
+}
+
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 

@@ -143,10 +143,12 @@ __global__ __launch_bounds__(256) void k_rho(DevParams p, int nbRho, const typen
                                              float *__restrict__ embDF, const unsigned *__restrict__ cl,
                                              const int *__restrict__ ncl, uint4 *__restrict__ bondC,
                                              uint4 *__restrict__ bondA) {
-    if ((int)blockIdx.x < nbRho)
-        rho_body<M, T, G>(p, blockIdx.x, pos, nlM, nnM, rho, embF, embDF);
+    const int b = blockIdx.x;  // nbRho is a multiple of 8: both ranges keep the XCD of each block
+    const bool xm = p.xcdMap & 2;
+    if (b < nbRho)
+        rho_body<M, T, G>(p, xm ? xcd_block(b, nbRho) : b, pos, nlM, nnM, rho, embF, embDF);
     else
-        bonds_body<M>(p, blockIdx.x - nbRho, pos, cl, ncl, bondC, bondA);
+        bonds_body<M>(p, xm ? xcd_block(b - nbRho, gridDim.x - nbRho) : b - nbRho, pos, cl, ncl, bondC, bondA);
 }
 
 // ---- metal pass -----------------------------------------------------------------------------
@@ -356,7 +358,9 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         dZ[b] = (float)(Bk[b].z - z0);
     }
     // For each bond k: k's own near neighbours m != i, the first two in k's list order (.cu:940-978).
-    for (int b = 0; b < counti; ++b) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {  // unrolled: every array index is a constant, nothing goes to scratch
+        if (b >= counti) break;
         const unsigned me[3] = {nbk[b].x, nbk[b].y, nbk[b].z};
         const int na = (int)nbk[b].w;
         P4 Bm[3];
@@ -374,6 +378,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         }
     }
     if (counti == 3) {  // SurfaceGrowth.cu:985-1061
+#pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float dr = (float)(absR[b] - R(1));
             const float fc = -p.mu_r * dr / (float)absR[b];
@@ -467,10 +472,14 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
                                                int integrate, DevState *st, sgk_record *records, int recCap) {
     using A = typename Prec<M>::A;
     Partial<A> q;
-    if ((int)blockIdx.x < nbM)
-        q = metal_body<M, T, G>(p, blockIdx.x, pos, vel, acc, nlM, nnM, embF, embDF, fric, integrate);
+    const int b = blockIdx.x;  // nbM is a multiple of 8: both ranges keep the XCD of each block
+    const bool xm = p.xcdMap & 4;
+    if (b < nbM)
+        q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF, fric,
+                                integrate);
     else
-        q = carbon_body<M, T>(p, blockIdx.x - nbM, pos, vel, acc, lj, nlj, bondC, bondA, integrate);
+        q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, gridDim.x - nbM) : b - nbM, pos, vel, acc, lj, nlj,
+                              bondC, bondA, integrate);
     const Partial<A> r = block_reduce_partial(q);
     __shared__ int last;
     if (threadIdx.x == 0) {
@@ -489,7 +498,6 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
 }
 
 // ---- launchers ---------------------------------------------------------------------------------
-static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <bool M, class T>
 static void rho_dispatch(sgk_ctx *c) {

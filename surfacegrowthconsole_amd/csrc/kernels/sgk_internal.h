@@ -38,10 +38,11 @@ struct sgk_ctx {
     void *acc = nullptr;   // R [3*nMol] (FP64 in the mixed form)
     float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;  // [nMolMe]
     // cells (two buffers, see the rebuild protocol in sgk_nebr.hip)
-    int *cellOf = nullptr;
+    int *cellOf[2] = {nullptr, nullptr};
     int *cnt[2] = {nullptr, nullptr};
     int *slots[2] = {nullptr, nullptr};
-    int *nonempty[2] = {nullptr, nullptr};
+    int *slotOf = nullptr;
+    float4 *cellPos = nullptr;
     // neighbour structures
     int *nlM = nullptr, *nnM = nullptr;                 // metal: row-major [nMolMe*nlCap]
     unsigned int *lj = nullptr, *cl = nullptr;          // carbon: column-major packed (idx<<8|pos)
@@ -58,6 +59,7 @@ struct sgk_ctx {
     int block = 256, metalLanes = 8, nbM = 0, nbC = 0, nbRho = 0, nbProps = 0;
     long long bytes = 0;
     bool haveList = false;
+    bool haveCells = false;                             // sgk_bin ran on the current positions
     int hostStep = 0;                                   // host copy of the step counter
     double lastRunMs = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

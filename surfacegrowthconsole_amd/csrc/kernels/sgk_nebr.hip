@@ -1,14 +1,14 @@
 // sgk_nebr.hip - cell binning, neighbour lists and the RDF histogram for gfx950.
 //
 // Rebuild protocol (two cell buffers, cur = state.cellbuf, nxt = cur ^ 1):
-//   binning     cell index per atom with the reference formula, slot by atomicAdd into nxt; the
-//               first atom of a cell appends it to nxt's non-empty list (bin_atom; in the step loop
-//               it runs inside the leapfrog kernel).
-//   k_sortcells one wave per non-empty cell of nxt sorts its slots ascending (bitonic, in
-//               registers), so traversals see atoms in index order; zeroes cur's counts for the
-//               rebuild after this one.
+//   binning     cell index per atom with the reference formula, slot by atomicAdd into nxt
+//               (bin_atom; in the step loop it runs inside the leapfrog kernel).
+//   k_sortcells per atom: the atom that took slot 0 of its cell has its wave sort that cell's
+//               slots ascending (bitonic, in registers) and write the cell-ordered float
+//               positions; every atom clears its cell's count in cur for the rebuild after this
+//               one.  No global counter is involved.
 //   k_nl        one thread per atom: the reference neighbour order from the sorted cells.
-//   commit      cellbuf = nxt; cur's non-empty count = 0 (the step reduction, or k_commit).
+//   commit      cellbuf = nxt (the step reduction, or k_commit).
 // The API path (sgk_bin) bins with k_bin.  In the device-resident loop the kernels return at once
 // unless state.nebrNow is set, so a captured step needs no host decision.
 //
@@ -75,43 +75,56 @@ __device__ __forceinline__ float shifted_rr(const float3 &a, const float3 &b, fl
 
 // BinAtomsIntoCellsK (SurfaceGrowth.cu:547-609) as an O(N) pass (API path).
 template <bool M>
-__global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos, int *__restrict__ cellOf,
-                      CellBufs bufs, DevState *st, int force) {
+__global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos, CellBufs bufs, DevState *st,
+                      int force) {
     if (!force && !st->nebrNow) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.nMol) return;
-    bin_atom(p, i, f3(pos[i]), cellOf, bufs, st);
+    bin_atom(p, i, f3(pos[i]), bufs, st);
 }
 
-// Sort each non-empty cell of the new buffer (<= 64 atoms, one per lane) ascending; zero the
-// counts of the outgoing buffer so it is clean for the rebuild after this one.
-__global__ void k_sortcells(DevParams p, CellBufs bufs, DevState *st, int force) {
+// Sort each occupied cell of the new buffer (<= 64 atoms, one per lane) ascending and store its
+// atoms' float positions in slot order; clear the counts of the outgoing buffer.  One thread per
+// atom; the owner of a cell is the atom holding slot 0, and the owner's wave sorts the cell.
+template <bool M>
+__global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                                   CellBufs bufs, DevState *st, int force) {
     if (!force && !st->nebrNow) return;
     const int b = st->cellbuf ^ 1, old = st->cellbuf;
     const int lane = threadIdx.x & 63;
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    const int ne = st->nonempty[b];
-    for (int e = wave; e < ne; e += nwaves) {
-        const int c = bufs.nonempty[b][e];
-        const int n = min(bufs.cnt[b][c], p.cellCap);
-        if (n <= 1) continue;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = n < p.nMol;  // every lane stays for the wave-wide sort
+    int cn = -1;
+    if (in) {
+        const int co = bufs.cellOf[old][n];
+        if (co >= 0) bufs.cnt[old][co] = 0;
+        cn = bufs.cellOf[b][n];
+    }
+    unsigned long long todo = __ballot(in && cn >= 0 && bufs.slotOf[n] == 0);
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        todo &= todo - 1ull;
+        const int c = __shfl(cn, l, 64);
+        const int cnt = min(bufs.cnt[b][c], p.cellCap);
         int *s = bufs.slots[b] + (size_t)c * p.cellCap;
-        int v = lane < n ? s[lane] : 0x7fffffff;
+        int v = lane < cnt ? s[lane] : 0x7fffffff;
+        if (cnt > 1) {
 #pragma unroll
-        for (int k = 2; k <= 64; k <<= 1) {
+            for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const int w = __shfl_xor(v, j, 64);
-                const bool lower = (lane & j) == 0, up = (lane & k) == 0;
-                v = (lower == up) ? min(v, w) : max(v, w);
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int w = __shfl_xor(v, j, 64);
+                    const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+                    v = (lower == up) ? min(v, w) : max(v, w);
+                }
             }
         }
-        if (lane < n) s[lane] = v;
+        if (lane < cnt) {
+            s[lane] = v;
+            const float3 q = f3(pos[v]);
+            bufs.cellPos[(size_t)c * p.cellCap + lane] = make_float4(q.x, q.y, q.z, 0.f);
+        }
     }
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
-    const int neo = st->nonempty[old];
-    for (int e = tid; e < neo; e += nth) bufs.cnt[old][bufs.nonempty[old][e]] = 0;
 }
 
 // BuildNebrListK (SurfaceGrowth.cu:612-711), one thread per atom over cells whose slots are sorted
@@ -124,20 +137,21 @@ __global__ void k_sortcells(DevParams p, CellBufs bufs, DevState *st, int force)
 //   keepFull:    fullNBL[j*nMol + i] = k, fullNN[i]      (reference layout)
 template <bool M>
 __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                            const int *__restrict__ cellOf, CellBufs bufs, DevState *st, int force,
+                                            CellBufs bufs, DevState *st, int force,
                                             int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
                                             unsigned *__restrict__ lj, int *__restrict__ nlj,
                                             unsigned *__restrict__ cl, int *__restrict__ ncl,
                                             int *__restrict__ fullNN, int *__restrict__ fullNBL) {
     if (!force && !st->nebrNow) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = ((p.xcdMap & 1) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= p.nMol) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
     const int *cnt = bufs.cnt[b];
     const int *slots = bufs.slots[b];
+    const float4 *cpos = bufs.cellPos;
     const bool metal = i < p.nMolMe;
     const int ic = i - p.nMolMe;
-    const int c = cellOf[i];
+    const int c = bufs.cellOf[b][i];
     int nn = 0, nljv = 0, nclv = 0, err = 0;
     if (c >= 0) {
         const int ccz = c / (p.cx * p.cy);
@@ -157,16 +171,18 @@ __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>:
             }
             const int n = min(ncur, p.cellCap);
             const int *s = slots + (size_t)C * p.cellCap;
+            const float4 *cp = cpos + (size_t)C * p.cellCap;
             for (int base = 0; base < n; base += 8) {
+                // slot indices and their positions are independent loads (cell-ordered copy)
                 const int4 k0 = *(const int4 *)(s + base);
                 const int4 k1 = *(const int4 *)(s + base + 4);
-                int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+                float4 P[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) P[t] = cp[base + t];
+                const int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
                 float3 B[8];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    if (base + t >= n) k[t] = i;  // slots past the count are not defined
-                    B[t] = f3(pos[k[t]]);
-                }
+                for (int t = 0; t < 8; ++t) B[t] = make_float3(P[t].x, P[t].y, P[t].z);
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     if (base + t >= n) continue;
@@ -219,9 +235,7 @@ __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>:
 
 // Make the freshly built buffer current (API path; the step path does this in its reduction).
 __global__ void k_commit(DevState *st) {
-    const int old = st->cellbuf;
-    st->cellbuf = old ^ 1;
-    st->nonempty[old] = 0;
+    st->cellbuf ^= 1;
     st->rebuilds += 1;
 }
 
@@ -230,13 +244,12 @@ __global__ void k_commit(DevState *st) {
 // Histogram in LDS, merged with one atomic per bin and block; counts are order independent.
 template <bool M>
 __global__ __launch_bounds__(256) void k_rdf(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                             const int *__restrict__ cellOf, CellBufs bufs, DevState *st,
-                                             unsigned *__restrict__ hist) {
+                                             CellBufs bufs, DevState *st, unsigned *__restrict__ hist) {
     extern __shared__ unsigned sh[];
     for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = i < p.nMolMe ? cellOf[i] : -1;
+    const int c = i < p.nMolMe ? bufs.cellOf[st->cellbuf][i] : -1;
     int edge = 0;
     if (c >= 0) {
         const int cb = st->cellbuf;
@@ -280,19 +293,24 @@ void launch_bin(sgk_ctx *c, bool force) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMol, 256);
     if (c->mixed)
-        hipLaunchKernelGGL(k_bin<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, (int)force);
+        hipLaunchKernelGGL(k_bin<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, cell_bufs(c),
+                           c->state, (int)force);
     else
-        hipLaunchKernelGGL(k_bin<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, (int)force);
+        hipLaunchKernelGGL(k_bin<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, cell_bufs(c),
+                           c->state, (int)force);
     profile_end(c, "bin");
 }
 
 void launch_sortcells(sgk_ctx *c, bool force) {
+    if (c->dp.nMol == 0) return;
     profile_begin(c);
-    const int waves = std::min<long long>(std::max(1, c->dp.nMol), 16384);
-    const unsigned g = blocks_for((long long)waves * 64, 256);
-    hipLaunchKernelGGL(k_sortcells, dim3(g), dim3(256), 0, c->stream, c->dp, cell_bufs(c), c->state, (int)force);
+    const unsigned g = blocks_for(c->dp.nMol, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_sortcells<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos,
+                           cell_bufs(c), c->state, (int)force);
+    else
+        hipLaunchKernelGGL(k_sortcells<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos,
+                           cell_bufs(c), c->state, (int)force);
     profile_end(c, "sortcells");
 }
 
@@ -302,12 +320,12 @@ void launch_nl(sgk_ctx *c, bool force) {
     const unsigned g = blocks_for(c->dp.nMol, 256);
     const int fromNext = force ? 0 : 1;  // step path: build from the new buffer before the commit
     if (c->mixed)
-        hipLaunchKernelGGL(k_nl<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
+        hipLaunchKernelGGL(k_nl<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, cell_bufs(c),
+                           c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
                            c->fullNN, c->fullNBL);
     else
-        hipLaunchKernelGGL(k_nl<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
+        hipLaunchKernelGGL(k_nl<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, cell_bufs(c),
+                           c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
                            c->fullNN, c->fullNBL);
     profile_end(c, "nl");
 }
@@ -322,11 +340,11 @@ void launch_rdf(sgk_ctx *c) {
     const unsigned g = blocks_for(c->dp.nMolMe, 256);
     const size_t shm = sizeof(unsigned) * (size_t)c->dp.sizeHistRdf;
     if (c->mixed)
-        hipLaunchKernelGGL(k_rdf<true>, dim3(g), dim3(256), shm, c->stream, c->dp, (const double4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, c->hist);
+        hipLaunchKernelGGL(k_rdf<true>, dim3(g), dim3(256), shm, c->stream, c->dp, (const double4 *)c->pos, cell_bufs(c),
+                           c->state, c->hist);
     else
-        hipLaunchKernelGGL(k_rdf<false>, dim3(g), dim3(256), shm, c->stream, c->dp, (const float4 *)c->pos, c->cellOf,
-                           cell_bufs(c), c->state, c->hist);
+        hipLaunchKernelGGL(k_rdf<false>, dim3(g), dim3(256), shm, c->stream, c->dp, (const float4 *)c->pos, cell_bufs(c),
+                           c->state, c->hist);
     profile_end(c, "rdf");
 }
 

@@ -31,6 +31,7 @@ struct DevParams {
     int cellCap, nlCap, ljCap, clCap, keepFull;
     int rebuildEvery;
     float dispBound;           // max dispHi for which the carbon close lists are exact
+    int xcdMap, binGroup;      // launch-shape options (sgk_config.xcd_map, bin_group)
 };
 
 // Per-block partial reductions written by the force kernels (or the props kernel).
@@ -45,7 +46,6 @@ struct DevState {
     int nebrNow;           // rebuild at the next step
     int errors;            // SGK_FLAG_* bits
     int cellbuf;           // cell buffer holding the current lists (0/1)
-    int nonempty[2];       // number of entries in each buffer's non-empty-cell list
     int rec;               // next record slot
     int rebuilds;          // rebuild counter
     double dispHi;

@@ -127,7 +127,6 @@ __device__ void finalize_body(const DevParams &p, const Partial<A> *part, int nP
     if (st->nebrNow) {  // this step rebuilt: the new cell buffer becomes current (SurfaceGrowth.cu:2193-2195)
         const int old = st->cellbuf;
         st->cellbuf = old ^ 1;
-        st->nonempty[old] = 0;
         st->rebuilds += 1;
         dispHi = A(0);
     }

@@ -19,7 +19,7 @@ template <bool M>
 __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
                                              typename Prec<M>::R *__restrict__ vel,
                                              const typename Prec<M>::R *__restrict__ acc,
-                                             int *__restrict__ cellOf, CellBufs bufs, DevState *st, int doBin) {
+                                             CellBufs bufs, DevState *st, int doBin) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
     if (r.z >= R(0.5f) * Lz) r.z -= Lz;
     else if (r.z < -R(0.5f) * Lz) r.z += Lz;
     pos[n] = r;
-    if (doBin && st->nebrNow) bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), cellOf, bufs, st);
+    if (doBin && st->nebrNow) bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
 }
 
 // LeapfrogStepK part 2 alone (the step loop fuses it into the force kernels).
@@ -177,10 +177,10 @@ void launch_lf1(sgk_ctx *c, bool bin) {
     const unsigned g = blocks_for(c->dp.nMol, 256);
     if (c->mixed)
         hipLaunchKernelGGL(k_lf1<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double4 *)c->pos, (double *)c->vel,
-                           (const double *)c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+                           (const double *)c->acc, cell_bufs(c), c->state, (int)bin);
     else
         hipLaunchKernelGGL(k_lf1<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos, (float *)c->vel,
-                           (const float *)c->acc, c->cellOf, cell_bufs(c), c->state, (int)bin);
+                           (const float *)c->acc, cell_bufs(c), c->state, (int)bin);
     profile_end(c, "lf1");
 }
 
