@@ -59,6 +59,15 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 }
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1/x in FP64: the hardware estimate refined by two Newton steps (within an ulp or two of the
+// correctly rounded quotient, at half the instructions of an IEEE division).
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Pair-math precision T (float, or double in the mixed form's FP64-pair variant).
@@ -73,7 +82,7 @@ struct PM<float> {
 };
 template <>
 struct PM<double> {
-    __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+    __device__ static __forceinline__ double rcp(double x) { return rcp64(x); }
     __device__ static __forceinline__ double sqrt(double x) { return ::sqrt(x); }
     __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
     __device__ static __forceinline__ double log(double x) { return ::log(x); }
@@ -149,36 +158,85 @@ __device__ __forceinline__ T lj_pair(const DevParams &p, T rr, T &u) {
     return T(48) * (T)p.epsLJ * rri3 * (rri3 - T(0.5)) * rri * (T)p.inv_sigma2;
 }
 
-// Wave64 reductions through DPP-backed shuffles.
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// Lane exchange through DPP (data-parallel primitives of the VALU, no LDS round trip).  Lanes in
+// rows that RM does not enable read `old`.  All lanes of the wave must be active.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_i32(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp(float old, float v) {
+    return __int_as_float(dpp_i32<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp(double old, double v) {
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = dpp_i32<CTRL, RM>((int)o, (int)x);
+    const int hi = dpp_i32<CTRL, RM>((int)(o >> 32), (int)(x >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// Full-row patterns (every lane has a source): no fallback value needed.
+template <int CTRL>
+__device__ __forceinline__ float dpp_all(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_all(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+enum : int {
+    kQuadSwap1 = 0xB1,  // quad_perm [1,0,3,2]
+    kQuadSwap2 = 0x4E,  // quad_perm [2,3,0,1]
+    kRowMirror = 0x140,
+    kRowHalfMirror = 0x141,
+    kRowBcast15 = 0x142,
+    kRowBcast31 = 0x143
+};
+
+struct OpSum {
+    template <class T>
+    __device__ static __forceinline__ T id() { return T(0); }
+    template <class T>
+    __device__ static __forceinline__ T f(T a, T b) { return a + b; }
+};
+struct OpMax {
+    template <class T>
+    __device__ static __forceinline__ T id() { return T(-3.0e38f); }
+    template <class T>
+    __device__ static __forceinline__ T f(T a, T b) { return b > a ? b : a; }
+};
+struct OpMin {
+    template <class T>
+    __device__ static __forceinline__ T id() { return T(3.0e38f); }
+    template <class T>
+    __device__ static __forceinline__ T f(T a, T b) { return b < a ? b : a; }
+};
+
+// Reduction over the wave; the result is valid in lane 63 (fixed order, so reproducible).
+template <class Op, class T>
+__device__ __forceinline__ T wave_reduce63(T v) {
+    const T e = Op::template id<T>();
+    v = Op::f(v, dpp_all<kQuadSwap1>(v));
+    v = Op::f(v, dpp_all<kQuadSwap2>(v));
+    v = Op::f(v, dpp_all<kRowHalfMirror>(v));
+    v = Op::f(v, dpp_all<kRowMirror>(v));
+    v = Op::f(v, dpp<kRowBcast15, 0xA>(e, v));
+    v = Op::f(v, dpp<kRowBcast31, 0xC>(e, v));
     return v;
 }
-template <class T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-template <class T>
-__device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-// Sum over aligned groups of G lanes (G a power of two <= 64).
+
+// Sum over aligned groups of G lanes (G a power of two <= 16); every lane of a group gets the
+// same value (each step adds a commutative pair).
 template <int G, class T>
 __device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group of 1..16 lanes");
+    if (G >= 2) v += dpp_all<kQuadSwap1>(v);
+    if (G >= 4) v += dpp_all<kQuadSwap2>(v);
+    if (G >= 8) v += dpp_all<kRowHalfMirror>(v);
+    if (G >= 16) v += dpp_all<kRowMirror>(v);
     return v;
 }
 

@@ -294,26 +294,30 @@ struct V3 {
     R x, y, z;
 };
 
+// Reciprocal in the displacement precision R.
+__device__ __forceinline__ float rcpR(float x) { return rcpf(x); }
+__device__ __forceinline__ double rcpR(double x) { return rcp64(x); }
+
 // Neighbour-angle terms of bond (i,k) with k's bonded neighbour m (SurfaceGrowth.cu:953-978).
 // rik: r_i - r_k, aik = |rik|; e: r_m - r_k, rr2 = |e|^2 (the reference divides by rr2 where |e|
 // is meant; kept).
 template <class R>
 __device__ __forceinline__ void neighbour_angle(const DevParams &p, const V3<R> &rik, R aik, const V3<R> &e, R rr2,
                                                 float mz_rel, float &fx, float &fy, float &fz) {
-    const R c = (rik.x * e.x + rik.y * e.y + rik.z * e.z) / (aik * rr2);
+    const R c = (rik.x * e.x + rik.y * e.y + rik.z * e.z) * rcpR(aik * rr2);
     float s;
     const float dth = AngleDev<R>::eval(c, p, s);
-    const float tmp = p.mu_theta * dth / s;
+    const float tmp = p.mu_theta * dth * rcpf(s);
     const float fa = (float)aik, frr = (float)rr2;
-    float fc = tmp / (fa * frr);
+    float fc = tmp * rcpf(fa * frr);
     fx += fc * (float)e.x;
     fy += fc * (float)e.y;
     fz += fc * (float)e.z;
-    fc = -tmp * (float)c / (fa * fa);
+    fc = -tmp * (float)c * rcpf(fa * fa);
     fx += fc * (float)rik.x;
     fy += fc * (float)rik.y;
     fz += fc * (float)rik.z;
-    fz += -p.mu_p * mz_rel / 9.f;
+    fz += -p.mu_p * mz_rel * (1.f / 9.f);
 }
 
 // ComputeForcesK carbon branch (SurfaceGrowth.cu:899-1062, 1083-1088), one thread per carbon.
@@ -381,7 +385,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float dr = (float)(absR[b] - R(1));
-            const float fc = -p.mu_r * dr / (float)absR[b];
+            const float fc = -p.mu_r * dr * rcpf((float)absR[b]);
             fx += fc * (float)rnn[b].x;
             fy += fc * (float)rnn[b].y;
             fz += fc * (float)rnn[b].z;
@@ -395,14 +399,15 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
 #pragma unroll
         for (int qq = 0; qq < 3; ++qq) {
             const int a = qq, b = qq == 2 ? 0 : qq + 1;
-            const R c = (rnn[a].x * rnn[b].x + rnn[a].y * rnn[b].y + rnn[a].z * rnn[b].z) / (absR[a] * absR[b]);
+            const R c = (rnn[a].x * rnn[b].x + rnn[a].y * rnn[b].y + rnn[a].z * rnn[b].z) * rcpR(absR[a] * absR[b]);
             float s;
             const float dth = AngleDev<R>::eval(c, p, s);
-            const float tmp = p.mu_theta * dth / s;
+            const float tmp = p.mu_theta * dth * rcpf(s);
             const float ra = (float)absR[a], rb = (float)absR[b], fcos = (float)c;
-            const float inv = 1.f / (ra * rb);
-            const float fcb = tmp * (1.f - ra * fcos / rb) * inv;
-            const float fca = tmp * (1.f - rb * fcos / ra) * inv;
+            const float ira = rcpf(ra), irb = rcpf(rb);
+            const float inv = ira * irb;
+            const float fcb = tmp * (1.f - ra * fcos * irb) * inv;
+            const float fca = tmp * (1.f - rb * fcos * ira) * inv;
             fx += fcb * (float)rnn[b].x + fca * (float)rnn[a].x;
             fy += fcb * (float)rnn[b].y + fca * (float)rnn[a].y;
             fz += fcb * (float)rnn[b].z + fca * (float)rnn[a].z;
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
     else
         q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, gridDim.x - nbM) : b - nbM, pos, vel, acc, lj, nlj,
                               bondC, bondA, integrate);
-    const Partial<A> r = block_reduce_partial(q);
+    const Partial<A> r = b < nbM ? block_reduce_partial<true>(q) : block_reduce_partial<false>(q);
     __shared__ int last;
     if (threadIdx.x == 0) {
         part[blockIdx.x] = r;

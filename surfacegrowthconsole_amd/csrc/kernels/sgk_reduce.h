@@ -34,30 +34,36 @@ __device__ __forceinline__ void partial_combine(Partial<T> &r, const Partial<T> 
     }
 }
 
-template <class T>
+// Wave reduction of a Partial (result in lane 63).  METAL = false reduces only the fields a
+// carbon atom contributes (energy, velocity sums, v^2 max); the others stay at their identity.
+template <bool METAL, class T>
 __device__ __forceinline__ void partial_wave(Partial<T> &q) {
-    q.u = wave_sum(q.u);
-    q.fr = wave_sum(q.fr);
-    q.vv = wave_sum(q.vv);
-    q.vmax = wave_max(q.vmax);
+    q.u = wave_reduce63<OpSum>(q.u);
+    q.vv = wave_reduce63<OpSum>(q.vv);
+    q.vmax = wave_reduce63<OpMax>(q.vmax);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        q.com[d] = wave_sum(q.com[d]);
-        q.cmv[d] = wave_sum(q.cmv[d]);
-        q.vs[d] = wave_sum(q.vs[d]);
-        q.bmin[d] = wave_min(q.bmin[d]);
-        q.bmax[d] = wave_max(q.bmax[d]);
+    for (int d = 0; d < 3; ++d) q.vs[d] = wave_reduce63<OpSum>(q.vs[d]);
+    if (METAL) {
+        q.fr = wave_reduce63<OpSum>(q.fr);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            q.com[d] = wave_reduce63<OpSum>(q.com[d]);
+            q.cmv[d] = wave_reduce63<OpSum>(q.cmv[d]);
+            q.bmin[d] = wave_reduce63<OpMin>(q.bmin[d]);
+            q.bmax[d] = wave_reduce63<OpMax>(q.bmax[d]);
+        }
     }
 }
 
-// Reduce one Partial per thread of the block; thread 0 returns the total (fixed order).
-template <class T>
+// Reduce one Partial per thread of the block; thread 0 returns the total (fixed order).  Every
+// thread of the block must call it.
+template <bool METAL = true, class T>
 __device__ Partial<T> block_reduce_partial(Partial<T> q) {
     __shared__ Partial<T> sh[16];
-    partial_wave(q);
+    partial_wave<METAL>(q);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     __syncthreads();
-    if (lane == 0) sh[w] = q;
+    if (lane == 63) sh[w] = q;
     __syncthreads();
     Partial<T> r = sh[0];
     if (threadIdx.x == 0)
