@@ -70,6 +70,43 @@ def measured_traffic(workload):
     return None
 
 
+def nve_drift(config, steps, oracle_steps):
+    """Total-energy drift without thermostat (eV/atom/ns, slope of a least-squares line) of each GPU
+    form over `steps` steps, and of the FP64 oracle over `oracle_steps` steps of the same start.
+    The reference's graphene energy is not the potential of its forces, so the drift is not zero
+    even in exact arithmetic: what a form adds to the FP64 oracle's drift is its precision cost."""
+    import numpy as np
+    import surfacegrowthconsole_amd as sg
+
+    s = sg.config_system(config, "config", stepThermostat=1 << 30)
+    p = s.params
+    dt_ns = p.deltaT * 0.2e-3  # time unit 0.2 ps
+    out = {"workload": config, "atoms": p.nMol, "steps": steps}
+
+    def slope(tot):
+        t = np.arange(1, len(tot) + 1) * dt_ns
+        return float(np.polyfit(t, np.asarray(tot) * p.enU, 1)[0])
+
+    for form in ("mixed", "mixed_fp32pairs", "fp32"):
+        with sg.Engine(p, rebuild_every=10, records_capacity=1000, **FORMS[form]) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            tot = []
+            while len(tot) < steps:
+                tot += [x["totEnergy"] for x in eng.run(min(1000, steps - len(tot)))]
+        out[form] = slope(tot)
+        out[form + "_first_" + str(oracle_steps)] = slope(tot[:oracle_steps])
+        log(f"[bench] NVE {form}: {out[form]:.4e} eV/atom/ns")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    o = oracle.Oracle(p, "f64")
+    err, _, _, _, recs = o.run(s.r, s.v, s.a, oracle_steps, 0, 1, 0.0, 10)
+    out["fp64_oracle_first_" + str(oracle_steps)] = slope([x["totEnergy"] for x in recs])
+    log(f"[bench] NVE FP64 oracle ({oracle_steps} steps): {out['fp64_oracle_first_' + str(oracle_steps)]:.4e}")
+    return out
+
+
 def b_alg(precision, f_me):
     return (96.0 if precision == "mixed" else 48.0) + 20.0 * f_me
 
@@ -110,10 +147,15 @@ def time_form(sys_, precision, steps, warmup, rebuild, chunk):
     return wall, dev_ms, stats, r
 
 
-def gpu_forces(params, r, precision):
+# Precision forms: plain FP32; mixed with FP64 pair arithmetic (default); mixed with FP32 pairs.
+FORMS = {"mixed": dict(precision="mixed"), "fp32": dict(precision="fp32"),
+         "mixed_fp32pairs": dict(precision="mixed", pair_fp64=0)}
+
+
+def gpu_forces(params, r, form):
     import surfacegrowthconsole_amd as sg
 
-    with sg.Engine(params, precision=precision) as eng:
+    with sg.Engine(params, **FORMS[form]) as eng:
         eng.upload(r)
         eng.bin()
         eng.build_nebr()
@@ -133,8 +175,8 @@ def force_errors(params, states):
     o64 = oracle.Oracle(params, "f64")
     out = {}
     for label, r in states.items():
-        for prec in ("mixed", "fp32"):
-            rin = np.asarray(r, np.float64) if prec == "mixed" else np.asarray(r, np.float32).astype(np.float64)
+        for prec in ("mixed", "mixed_fp32pairs", "fp32"):
+            rin = np.asarray(r, np.float64) if prec != "fp32" else np.asarray(r, np.float32).astype(np.float64)
             _, nn, nbl = o64.neighbour_list(rin)
             a64, _, _ = o64.forces(rin, nn, nbl)
             acc = gpu_forces(params, rin, prec)
@@ -176,9 +218,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="run the copy kernel for counter calibration")
+    ap.add_argument("--nve", default=None, choices=["small", "medium", "large", "icosa"],
+                    help="measure the NVE energy drift of every form on this workload instead")
+    ap.add_argument("--nve-steps", type=int, default=5000)
+    ap.add_argument("--nve-oracle-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
+        return
+    if args.nve:
+        build_if_needed()
+        print(json.dumps({"nve_drift_eV_per_atom_per_ns": nve_drift(args.nve, args.nve_steps, args.nve_oracle_steps)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

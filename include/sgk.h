@@ -167,8 +167,21 @@ int sgk_kernel_stats(sgk_ctx *ctx, sgk_kernel_stat *stats, int max);
 long long sgk_device_bytes(sgk_ctx *ctx);
 /* sizeof of sgk_config, sgk_record, sgk_kernel_stat (for bindings); returns the number written. */
 int sgk_struct_sizes(int *out, int max);
-/* Lanes per metal atom chosen at creation. */
+/* Lanes per metal atom in use (chosen at creation, or by sgk_autotune). */
 int sgk_metal_lanes(const sgk_ctx *ctx);
+
+/* Autotuner (no reference counterpart: the reference has fixed launch shapes).  Times the density
+ * and force passes on the current state for every candidate launch shape - lanes per metal atom
+ * 1/2/4/8/16 and the XCD map of those passes on or off - `reps` launches each, and keeps the
+ * fastest.  The passes only rewrite quantities they recompute identically (acceleration, per-atom
+ * energy, friction), so the state is unchanged.  Needs a valid neighbour list.  Results differ
+ * between shapes only by the FP summation order of the metal neighbour loop. */
+typedef struct sgk_tune_entry {
+    int metal_lanes;
+    int xcd_map;
+    double us;            /* mean time of one density + force evaluation */
+} sgk_tune_entry;
+int sgk_autotune(sgk_ctx *ctx, int reps, sgk_tune_entry *entries, int max_entries, int *n_entries);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,10 @@ class Material(C.Structure):
         ("Fn", _f * 4), ("F", _f * 4), ("eta", _f), ("Fe", _f), ("massMe", _f), ("density", _f)]
 
 
+class SgkTuneEntry(C.Structure):
+    _fields_ = [("metal_lanes", C.c_int), ("xcd_map", C.c_int), ("us", C.c_double)]
+
+
 class SgkConfig(C.Structure):
     _fields_ = [
         ("precision", _i), ("nl_capacity", _i), ("cell_capacity", _i), ("lj_capacity", _i),
@@ -141,7 +145,7 @@ EXPORTED_SYMBOLS = [
     "sgk_compute_rho", "sgk_compute_forces", "sgk_get_rho", "sgk_get_forces", "sgk_leapfrog",
     "sgk_thermostat", "sgk_eval_props", "sgk_rdf_clear", "sgk_rdf_accumulate", "sgk_rdf_get",
     "sgk_run", "sgk_last_run_ms", "sgk_set_profiling", "sgk_kernel_stats", "sgk_device_bytes",
-    "sgk_struct_sizes", "sgk_metal_lanes",
+    "sgk_struct_sizes", "sgk_metal_lanes", "sgk_autotune",
 ]
 
 _lib = None
@@ -199,6 +203,7 @@ def _proto(lib):
         "sgk_device_bytes": (C.c_longlong, [vp]),
         "sgk_struct_sizes": (_i, [ip, _i]),
         "sgk_metal_lanes": (_i, [vp]),
+        "sgk_autotune": (_i, [vp, _i, vp, _i, vp]),
     }
     for name, (res, args) in spec.items():
         fn = getattr(lib, name)

@@ -330,7 +330,10 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
         SGK_TRY(dalloc(c, &c->fullNBL, nMol * (size_t)p.nlCap, false));
     }
-    const size_t nParts = std::max<size_t>((size_t)(c->nbM + c->nbC), (size_t)c->nbProps);
+    // partials for the largest force grid any launch shape gives (16 lanes per metal atom)
+    const size_t nbMmax = (size_t)((((long long)p.nMolMe * 16 + 255) / 256 + 7) / 8 * 8);
+    const size_t nParts = std::max<size_t>(std::max<size_t>(nbMmax, (size_t)c->nbM) + (size_t)c->nbC,
+                                           (size_t)c->nbProps);
     SGK_TRY(dalloc(c, (char **)&c->partials, nParts * sgk::partial_bytes(c), true));
     SGK_TRY(dalloc(c, &c->state, 1, true));
     SGK_TRY(dalloc(c, &c->records, (size_t)cfg.records_capacity, true));
@@ -848,6 +851,74 @@ int sgk_struct_sizes(int *out, int max) {
     int n = 0;
     for (; n < max && n < 3; ++n) out[n] = s[n];
     return n;
+}
+
+static void set_metal_lanes(sgk_ctx *c, int G) {
+    c->metalLanes = G;
+    c->nbM = (int)((((long long)c->dp.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+    c->nbRho = c->nbM;
+}
+
+static void drop_graphs(sgk_ctx *c) {
+    for (auto &g : c->graphs)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    c->graphs.clear();
+}
+
+int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, int *nEntries) {
+    if (!c) return SGK_E_ARG;
+    if (!c->haveList) return fail(c, SGK_E_STATE, "no neighbour list: call sgk_bin and sgk_build_nebr (or sgk_run) first");
+    if (reps <= 0) reps = 20;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    const int lanes[] = {1, 2, 4, 8, 16};
+    const int maps[] = {c->dp.xcdMap & ~6, c->dp.xcdMap | 6};
+    int bestG = c->metalLanes, bestMap = c->dp.xcdMap, n = 0;
+    double best = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SGK_HIP(c, hipEventCreate(&e0));
+    SGK_HIP(c, hipEventCreate(&e1));
+    for (int G : lanes) {
+        if (c->dp.nMolMe == 0 && G != lanes[0]) break;  // no metal pass to shape
+        for (int m : maps) {
+            set_metal_lanes(c, G);
+            c->dp.xcdMap = m;
+            sgk::launch_rho(c);  // warm-up
+            sgk::launch_force(c, false, false);
+            SGK_HIP(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; ++r) {
+                sgk::launch_rho(c);
+                sgk::launch_force(c, false, false);
+            }
+            SGK_HIP(c, hipEventRecord(e1, c->stream));
+            SGK_HIP(c, hipGetLastError());
+            SGK_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            SGK_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1e3 * ms / reps;
+            if (entries && n < maxEntries) entries[n] = sgk_tune_entry{G, m, us};
+            ++n;
+            if (us < best) {
+                best = us;
+                bestG = G;
+                bestMap = m;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_metal_lanes(c, bestG);
+    c->dp.xcdMap = bestMap;
+    c->cfg.metal_lanes = bestG;
+    c->cfg.xcd_map = bestMap;
+    // leave acceleration, energy and friction as the chosen shape computes them
+    sgk::launch_rho(c);
+    sgk::launch_force(c, false, false);
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured launches carry the old grid and parameters
+    c->profiling = prof;
+    if (nEntries) *nEntries = n;
+    return check_flags(c);
 }
 
 int sgk_metal_lanes(const sgk_ctx *c) { return c ? c->metalLanes : 0; }

@@ -239,6 +239,14 @@ class Engine:
     def set_profiling(self, on):
         self._check(self.lib.sgk_set_profiling(self.h, int(on)), "sgk_set_profiling")
 
+    def autotune(self, reps=20):
+        """Pick the fastest launch shape of the density and force passes on the current state;
+        returns [(metal_lanes, xcd_map, us), ...] for every candidate."""
+        buf = (N.SgkTuneEntry * 16)()
+        n = C.c_int()
+        self._check(self.lib.sgk_autotune(self.h, reps, buf, 16, C.byref(n)), "sgk_autotune")
+        return [(e.metal_lanes, e.xcd_map, e.us) for e in buf[: n.value]]
+
     def kernel_stats(self):
         n = self.lib.sgk_kernel_stats(self.h, None, 0)
         arr = (N.SgkKernelStat * max(n, 1))()

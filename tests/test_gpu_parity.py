@@ -240,3 +240,37 @@ def test_graph_and_eager_bitwise_identical(gpu):
     for x in out[1:]:
         for a, b in zip(out[0], x):
             assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_results(gpu):
+    """Every launch shape the autotuner tries gives the default shape's forces up to the FP
+    summation order of the metal neighbour loop; the tuned engine keeps stepping correctly."""
+    s = system("medium")
+    p = s.params
+    for prec, tol in (("mixed", 1e-12), ("fp32", 2e-6)):
+        print(f"[parity] autotune {prec}", flush=True)
+        with sg.Engine(p, precision=prec) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.bin()
+            eng.build_nebr()
+            eng.compute_forces()
+            a0, e0, _ = eng.forces()
+            table = eng.autotune(reps=3)
+            assert len(table) == 10 and all(us > 0 for _, _, us in table)
+            a1, e1, _ = eng.forces()
+            scale = np.abs(a0).max()
+            assert np.abs(a1 - a0).max() <= tol * scale
+            assert abs(e1.sum() - e0.sum()) <= tol * np.abs(e0).sum()
+            lanes = eng.metal_lanes
+            assert lanes in (1, 2, 4, 8, 16)
+            eng.upload(s.r, s.v, s.a)  # the run starts from the initial state (zero acceleration)
+            eng.set_run_state(0, 1, 0.0)
+            recs = eng.run(30)
+        with sg.Engine(p, precision=prec, metal_lanes=lanes) as ref:
+            ref.upload(s.r, s.v, s.a)
+            ref.set_run_state(0, 1, 0.0)
+            recs_ref = ref.run(30)
+        a = np.array([x["totEnergy"] for x in recs])
+        b = np.array([x["totEnergy"] for x in recs_ref])
+        # the XCD map regroups the block partials: FP summation order of the energy sums only
+        assert np.abs(a - b).max() <= (1e-9 if prec == "mixed" else 1e-6) * np.abs(b).max()
