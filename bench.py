@@ -19,6 +19,7 @@ GPU forms, on the initial state and on the state the timed mixed run ends in.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -105,6 +106,77 @@ def nve_drift(config, steps, oracle_steps):
     out["fp64_oracle_first_" + str(oracle_steps)] = slope([x["totEnergy"] for x in recs])
     log(f"[bench] NVE FP64 oracle ({oracle_steps} steps): {out['fp64_oracle_first_' + str(oracle_steps)]:.4e}")
     return out
+
+
+def batch_workload(nrep=64, seed=2024):
+    """The uneven-replica ensemble: Ni fcc spheres of log-uniform size in [55, 2000] atoms, each on a
+    graphene sheet whose sides are at least 3 cluster diameters (2,016-16,128 C, 32 ux uy atoms),
+    temperatures uniform in [300, 1200] K; config variant otherwise."""
+    import numpy as np
+    import surfacegrowthconsole_amd as sg
+
+    rng = np.random.default_rng(seed)
+    a = 3.5196  # Ni lattice constant, angstrom
+    out = []
+    for k in range(nrep):
+        n = int(round(math.exp(rng.uniform(math.log(55), math.log(2000)))))
+        d = 2.0 * a * (3.0 * n / (16.0 * math.pi)) ** (1.0 / 3.0)
+        ux = max(2, math.ceil(3.0 * d / (8 * math.cos(math.pi / 6) * 1.42)))
+        uy = max(2, math.ceil(3.0 * d / (6 * 1.42)))
+        # sheet area log-uniform between the smallest allowed and 16,128 atoms (ux*uy <= 504)
+        cells = math.exp(rng.uniform(math.log(max(63, ux * uy)), math.log(504)))
+        while ux * uy < cells and (ux + 1) * uy <= 504:
+            ux += 1
+            if ux * uy < cells and ux * (uy + 1) <= 504:
+                uy += 1
+        t = float(rng.uniform(300.0, 1200.0))
+        out.append(sg.config_system("small", "config", ux=ux, uy=uy, n_metal=n, temperature=t, seed=100 + k))
+    return out
+
+
+def bench_batch(steps, warmup, nrep):
+    """Atom-steps/s of the whole ensemble batched (one launch per stage) and, for comparison, of
+    the same replicas advanced one after another, each in its own context."""
+    import surfacegrowthconsole_amd as sg
+
+    systems = batch_workload(nrep)
+    natoms = sum(s.params.nMol for s in systems)
+    log(f"[bench] batch: {nrep} replicas, {natoms} atoms")
+    res = {"workload": f"{nrep} uneven replicas (55-2000 Ni, 2016-16128 C, 300-1200 K)", "replicas": nrep,
+           "atoms": natoms, "steps": steps}
+    for prec in ("mixed", "fp32"):
+        with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, records_capacity=500) as bt:
+            for k, s in enumerate(systems):
+                bt.upload(k, s.r, s.v, s.a)
+                bt.set_run_state(k, 0, 1, 0.0)
+            bt.run(warmup, records=False)
+            t0 = time.perf_counter()
+            done = 0
+            while done < steps:
+                n = min(500, steps - done)
+                bt.run(n, records=False)
+                done += n
+            wall = time.perf_counter() - t0
+        res[f"batched_{prec}"] = natoms * steps / wall
+        log(f"[bench] batch {prec}: {wall / steps * 1e3:.3f} ms/step, {natoms * steps / wall:.4e} atom-steps/s")
+        # one context per replica, advanced in turn
+        engines = []
+        for s in systems:
+            e = sg.Engine(s.params, precision=prec, rebuild_every=10, records_capacity=500)
+            e.upload(s.r, s.v, s.a)
+            e.set_run_state(0, 1, 0.0)
+            e.run(warmup, records=False)
+            engines.append(e)
+        seq_steps = max(50, steps // 10)
+        t0 = time.perf_counter()
+        for e in engines:
+            e.run(seq_steps, records=False)
+        wall = time.perf_counter() - t0
+        for e in engines:
+            e.close()
+        res[f"sequential_{prec}"] = natoms * seq_steps / wall
+        log(f"[bench] one context per replica {prec}: {natoms * seq_steps / wall:.4e} atom-steps/s")
+    return res
 
 
 def b_alg(precision, f_me):
@@ -221,10 +293,16 @@ def main():
     ap.add_argument("--nve", default=None, choices=["small", "medium", "large", "icosa"],
                     help="measure the NVE energy drift of every form on this workload instead")
     ap.add_argument("--nve-steps", type=int, default=5000)
+    ap.add_argument("--batch", type=int, default=0, metavar="NREP",
+                    help="time an ensemble of NREP uneven replicas, batched and one context each")
     ap.add_argument("--nve-oracle-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
+        return
+    if args.batch:
+        build_if_needed()
+        print(json.dumps({"batch": bench_batch(args.steps, args.warmup, args.batch)}))
         return
     if args.nve:
         build_if_needed()

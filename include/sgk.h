@@ -183,6 +183,31 @@ typedef struct sgk_tune_entry {
 } sgk_tune_entry;
 int sgk_autotune(sgk_ctx *ctx, int reps, sgk_tune_entry *entries, int max_entries, int *n_entries);
 
+/* ---- batched replicas (no reference counterpart: the reference runs one system per process) ----
+ * Many replicas of uneven size (cluster size, sheet size, temperature) advance in lockstep with one
+ * launch per stage for all of them, steps captured into hipGraphs.  Replicas share the potentials,
+ * time step, neighbour-list shell and capacity, thermostat schedule and RDF bins (checked at
+ * creation); geometry, atom counts and temperature are free.  Each replica keeps its own cells,
+ * lists, step state, reductions and records, and its results equal those of the same replica run
+ * alone in an sgk_ctx with xcd_map = 0 and the same lanes per metal atom. */
+typedef struct sgk_batch sgk_batch;
+int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfg, sgk_batch **out);
+int sgk_batch_destroy(sgk_batch *b);
+const char *sgk_batch_last_error(const sgk_batch *b);
+/* State of one replica, reference layouts: r[4*n] (x,y,z,E), v[3*n], a[3*n]; any may be NULL. */
+int sgk_batch_upload_f64(sgk_batch *b, int rep, const double *r, const double *v, const double *a);
+int sgk_batch_download_f64(sgk_batch *b, int rep, double *r, double *v, double *a);
+/* Step counter (replica 0's is the batch's), rebuild flag and displacement bound of one replica. */
+int sgk_batch_set_run_state(sgk_batch *b, int rep, int stepCount, int nebrNow, double dispHi);
+/* nsteps steps of every replica; records (may be NULL) receives nrep*nsteps records, replica-major. */
+int sgk_batch_run(sgk_batch *b, int nsteps, int rdfEvery, sgk_record *records);
+double sgk_batch_last_run_ms(const sgk_batch *b);
+int sgk_batch_rdf_get(sgk_batch *b, int rep, unsigned int *hist);
+int sgk_batch_rdf_clear(sgk_batch *b);
+long long sgk_batch_device_bytes(const sgk_batch *b);
+int sgk_batch_metal_lanes(const sgk_batch *b);
+int sgk_batch_atoms(const sgk_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,6 +146,10 @@ EXPORTED_SYMBOLS = [
     "sgk_thermostat", "sgk_eval_props", "sgk_rdf_clear", "sgk_rdf_accumulate", "sgk_rdf_get",
     "sgk_run", "sgk_last_run_ms", "sgk_set_profiling", "sgk_kernel_stats", "sgk_device_bytes",
     "sgk_struct_sizes", "sgk_metal_lanes", "sgk_autotune",
+    "sgk_batch_create", "sgk_batch_destroy", "sgk_batch_last_error", "sgk_batch_upload_f64",
+    "sgk_batch_download_f64", "sgk_batch_set_run_state", "sgk_batch_run", "sgk_batch_last_run_ms",
+    "sgk_batch_rdf_get", "sgk_batch_rdf_clear", "sgk_batch_device_bytes", "sgk_batch_metal_lanes",
+    "sgk_batch_atoms",
 ]
 
 _lib = None
@@ -204,6 +208,19 @@ def _proto(lib):
         "sgk_struct_sizes": (_i, [ip, _i]),
         "sgk_metal_lanes": (_i, [vp]),
         "sgk_autotune": (_i, [vp, _i, vp, _i, vp]),
+        "sgk_batch_create": (_i, [_i, vp, P(SgkConfig), P(vp)]),
+        "sgk_batch_destroy": (_i, [vp]),
+        "sgk_batch_last_error": (C.c_char_p, [vp]),
+        "sgk_batch_upload_f64": (_i, [vp, _i, vp, vp, vp]),
+        "sgk_batch_download_f64": (_i, [vp, _i, vp, vp, vp]),
+        "sgk_batch_set_run_state": (_i, [vp, _i, _i, _i, C.c_double]),
+        "sgk_batch_run": (_i, [vp, _i, _i, vp]),
+        "sgk_batch_last_run_ms": (C.c_double, [vp]),
+        "sgk_batch_rdf_get": (_i, [vp, _i, vp]),
+        "sgk_batch_rdf_clear": (_i, [vp]),
+        "sgk_batch_device_bytes": (C.c_longlong, [vp]),
+        "sgk_batch_metal_lanes": (_i, [vp]),
+        "sgk_batch_atoms": (_i, [vp]),
     }
     for name, (res, args) in spec.items():
         fn = getattr(lib, name)

@@ -128,9 +128,7 @@ void profile_end(sgk_ctx *c, const char *name) {
 
 }  // namespace sgk
 
-static void fill_dev_params(sgk_ctx *c) {
-    const SimParams &s = c->sp;
-    DevParams &p = c->dp;
+void sgk::make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &p) {
     std::memset(&p, 0, sizeof(p));
     p.nMol = s.nMol;
     p.nMolMe = s.nMolMe;
@@ -150,7 +148,7 @@ static void fill_dev_params(sgk_ctx *c) {
     p.rrCutEam = s.rrCutEam;
     p.rrCutLJ = s.rrCutLJ;
     p.rrCutC = s.rrCutC;
-    const float margin = c->cfg.close_margin;
+    const float margin = cfg.close_margin;
     const float closeR = s.rCutC + 2.f * (0.5f * s.rNebrShell + margin);
     p.closeR2 = closeR * closeR;
     p.dispBound = 0.5f * s.rNebrShell + margin;
@@ -198,14 +196,14 @@ static void fill_dev_params(sgk_ctx *c) {
     p.intervalRdf = s.intervalRdf;
     p.sizeHistRdf = s.sizeHistRdf;
     p.deltaF = s.deltaF;
-    p.cellCap = c->cfg.cell_capacity;
-    p.nlCap = c->cfg.nl_capacity;
-    p.ljCap = c->cfg.lj_capacity;
-    p.clCap = c->cfg.close_capacity;
-    p.keepFull = c->cfg.keep_full_nl;
-    p.rebuildEvery = c->cfg.rebuild_every;
-    p.xcdMap = c->cfg.xcd_map;
-    p.binGroup = c->cfg.bin_group;
+    p.cellCap = cfg.cell_capacity;
+    p.nlCap = cfg.nl_capacity;
+    p.ljCap = cfg.lj_capacity;
+    p.clCap = cfg.close_capacity;
+    p.keepFull = cfg.keep_full_nl;
+    p.rebuildEvery = cfg.rebuild_every;
+    p.xcdMap = cfg.xcd_map;
+    p.binGroup = cfg.bin_group;
 }
 
 extern "C" {
@@ -285,7 +283,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
             return SGK_E_HIP;
         }
     }
-    fill_dev_params(c);
+    sgk::make_dev_params(c->sp, c->cfg, c->dp);
     const DevParams &p = c->dp;
     // lanes per metal atom: enough waves to cover the machine for small clusters
     int G = cfg.metal_lanes;

@@ -7,18 +7,7 @@
 
 namespace sgk {
 
-// Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).
-// cellOf[b][n] is atom n's cell in buffer b (-1: not binned); slotOf[n] its slot in the buffer
-// being built; cellPos[c*cap + s] the float position of the atom in slot s of cell c after the sort
-// (built with the buffer, read by the neighbour-list pass).
-struct CellBufs {
-    int *cnt[2];
-    int *slots[2];
-    int *cellOf[2];
-    int *slotOf;
-    float4 *cellPos;
-};
-
+// The cell buffers of a single-system context.
 inline CellBufs cell_bufs(sgk_ctx *c) {
     CellBufs b;
     for (int k = 0; k < 2; ++k) {

@@ -18,6 +18,7 @@
 // and angle deviations are formed in the storage precision R (FP64 in the mixed form).  EAM and LJ
 // pair arithmetic runs in T (FP32, or FP64 with pair_fp64); graphene force-law arithmetic is FP32.
 // Gathers are issued in chunks of predicated loads so several are in flight per thread.
+#include "sgk_batch.h"
 #include "sgk_device.h"
 #include "sgk_internal.h"
 #include "sgk_reduce.h"
@@ -136,6 +137,20 @@ __device__ __forceinline__ void bonds_body(const DevParams &p, int blk, const ty
     bondA[ic] = make_uint4(ab[0], ab[1], ab[2], (unsigned)na);
 }
 
+// One workgroup of the density launch: blocks [0, nbRho) metal density, the rest carbon near lists.
+template <bool M, class T, int G>
+__device__ __forceinline__ void rho_block(const DevParams &p, int b, int nbRho, int nb, bool xm,
+                                          const typename Prec<M>::P4 *__restrict__ pos, const int *__restrict__ nlM,
+                                          const int *__restrict__ nnM, float *__restrict__ rho,
+                                          float *__restrict__ embF, float *__restrict__ embDF,
+                                          const unsigned *__restrict__ cl, const int *__restrict__ ncl,
+                                          uint4 *__restrict__ bondC, uint4 *__restrict__ bondA) {
+    if (b < nbRho)
+        rho_body<M, T, G>(p, xm ? xcd_block(b, nbRho) : b, pos, nlM, nnM, rho, embF, embDF);
+    else
+        bonds_body<M>(p, xm ? xcd_block(b - nbRho, nb - nbRho) : b - nbRho, pos, cl, ncl, bondC, bondA);
+}
+
 template <bool M, class T, int G>
 __global__ __launch_bounds__(256) void k_rho(DevParams p, int nbRho, const typename Prec<M>::P4 *__restrict__ pos,
                                              const int *__restrict__ nlM, const int *__restrict__ nnM,
@@ -143,12 +158,20 @@ __global__ __launch_bounds__(256) void k_rho(DevParams p, int nbRho, const typen
                                              float *__restrict__ embDF, const unsigned *__restrict__ cl,
                                              const int *__restrict__ ncl, uint4 *__restrict__ bondC,
                                              uint4 *__restrict__ bondA) {
-    const int b = blockIdx.x;  // nbRho is a multiple of 8: both ranges keep the XCD of each block
-    const bool xm = p.xcdMap & 2;
-    if (b < nbRho)
-        rho_body<M, T, G>(p, xm ? xcd_block(b, nbRho) : b, pos, nlM, nnM, rho, embF, embDF);
-    else
-        bonds_body<M>(p, xm ? xcd_block(b - nbRho, gridDim.x - nbRho) : b - nbRho, pos, cl, ncl, bondC, bondA);
+    // nbRho is a multiple of 8: both ranges keep the XCD of each block
+    rho_block<M, T, G>(p, blockIdx.x, nbRho, gridDim.x, p.xcdMap & 2, pos, nlM, nnM, rho, embF, embDF, cl, ncl,
+                       bondC, bondA);
+}
+
+template <bool M, class T, int G>
+__global__ __launch_bounds__(256) void kb_rho(BatchArgs a) {
+    const int2 e = a.forceTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const DevParams &p = B.p;
+    rho_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (const typename Prec<M>::P4 *)a.pos + B.atomOff,
+                       a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.rho + B.metalOff, a.embF + B.metalOff,
+                       a.embDF + B.metalOff, a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff,
+                       a.bondC + B.carbonOff, a.bondA + B.carbonOff);
 }
 
 // ---- metal pass -----------------------------------------------------------------------------
@@ -465,7 +488,43 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     return q;
 }
 
-// One launch for both passes; the last block to finish reduces the partials of all blocks.
+// One workgroup of the force launch (blocks [0, nbM) metal, [nbM, nb) carbon).  With st set, the
+// last of the nb workgroups to finish reduces the nb partials and runs the step bookkeeping.
+template <bool M, class T, int G>
+__device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, int nb, bool xm,
+                                            typename Prec<M>::P4 *__restrict__ pos,
+                                            typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
+                                            const int *__restrict__ nlM, const int *__restrict__ nnM,
+                                            const float *__restrict__ embF, const float *__restrict__ embDF,
+                                            float *__restrict__ fric, const unsigned *__restrict__ lj,
+                                            const int *__restrict__ nlj, const uint4 *__restrict__ bondC,
+                                            const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
+                                            int integrate, DevState *st, sgk_record *records, int recCap) {
+    using A = typename Prec<M>::A;
+    Partial<A> q;
+    if (b < nbM)
+        q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF, fric,
+                                integrate);
+    else
+        q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, vel, acc, lj, nlj, bondC, bondA,
+                              integrate);
+    const Partial<A> r = b < nbM ? block_reduce_partial<true>(q) : block_reduce_partial<false>(q);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        part[b] = r;
+        last = 0;
+        if (st) {
+            __threadfence();
+            last = atomicAdd(&st->blocksDone, 1u) == (unsigned)nb - 1u;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    finalize_body<A>(p, part, nb, st, records, recCap, 0);
+    if (threadIdx.x == 0) st->blocksDone = 0;
+}
+
 template <bool M, class T, int G>
 __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
                                                typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
@@ -475,31 +534,25 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
                                                const int *__restrict__ nlj, const uint4 *__restrict__ bondC,
                                                const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
                                                int integrate, DevState *st, sgk_record *records, int recCap) {
+    // nbM is a multiple of 8: both ranges keep the XCD of each block
+    force_block<M, T, G>(p, blockIdx.x, nbM, gridDim.x, p.xcdMap & 4, pos, vel, acc, nlM, nnM, embF, embDF, fric,
+                         lj, nlj, bondC, bondA, part, integrate, st, records, recCap);
+}
+
+// Batched: each replica's workgroups reduce into its own partials, state and records.
+template <bool M, class T, int G>
+__global__ __launch_bounds__(256) void kb_force(BatchArgs a) {
+    using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
-    Partial<A> q;
-    const int b = blockIdx.x;  // nbM is a multiple of 8: both ranges keep the XCD of each block
-    const bool xm = p.xcdMap & 4;
-    if (b < nbM)
-        q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF, fric,
-                                integrate);
-    else
-        q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, gridDim.x - nbM) : b - nbM, pos, vel, acc, lj, nlj,
-                              bondC, bondA, integrate);
-    const Partial<A> r = b < nbM ? block_reduce_partial<true>(q) : block_reduce_partial<false>(q);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = r;
-        last = 0;
-        if (st) {
-            __threadfence();
-            last = atomicAdd(&st->blocksDone, 1u) == gridDim.x - 1;
-        }
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    finalize_body<A>(p, part, gridDim.x, st, records, recCap, 0);
-    if (threadIdx.x == 0) st->blocksDone = 0;
+    const int2 e = a.forceTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const DevParams &p = B.p;
+    force_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (typename Prec<M>::P4 *)a.pos + B.atomOff,
+                         (R *)a.vel + 3 * B.atomOff, (R *)a.acc + 3 * B.atomOff, a.nlM + B.metalOff * p.nlCap,
+                         a.nnM + B.metalOff, a.embF + B.metalOff, a.embDF + B.metalOff, a.fric + B.metalOff,
+                         a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.bondC + B.carbonOff,
+                         a.bondA + B.carbonOff, (Partial<A> *)a.partials + B.partOff, 1, a.state + e.x,
+                         a.records + (size_t)e.x * a.recCap, a.recCap);
 }
 
 // ---- launchers ---------------------------------------------------------------------------------
@@ -559,6 +612,39 @@ void launch_rho(sgk_ctx *c) {
         rho_dispatch<true, float>(c);
     profile_end(c, "rho");
 }
+
+template <bool M, class T>
+static void batch_dispatch(sgk_batch *b, bool force) {
+    const dim3 g(b->nForceBlocks), t(256);
+#define SGK_B(GG)                                                                            \
+    do {                                                                                     \
+        if (force)                                                                           \
+            hipLaunchKernelGGL((kb_force<M, T, GG>), g, t, 0, b->stream, b->args());         \
+        else                                                                                 \
+            hipLaunchKernelGGL((kb_rho<M, T, GG>), g, t, 0, b->stream, b->args());           \
+    } while (0)
+    switch (b->metalLanes) {
+        case 1: SGK_B(1); break;
+        case 2: SGK_B(2); break;
+        case 4: SGK_B(4); break;
+        case 16: SGK_B(16); break;
+        default: SGK_B(8); break;
+    }
+#undef SGK_B
+}
+
+static void batch_launch(sgk_batch *b, bool force) {
+    if (b->nForceBlocks == 0) return;
+    if (!b->mixed)
+        batch_dispatch<false, float>(b, force);
+    else if (b->cfg.pair_fp64)
+        batch_dispatch<true, double>(b, force);
+    else
+        batch_dispatch<true, float>(b, force);
+}
+
+void launch_batch_rho(sgk_batch *b) { batch_launch(b, false); }
+void launch_batch_force(sgk_batch *b) { batch_launch(b, true); }
 
 void launch_force(sgk_ctx *c, bool integrate, bool finalize) {
     if (c->nbM + c->nbC == 0) return;

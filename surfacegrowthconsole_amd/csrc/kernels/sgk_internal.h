@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "sgk.h"
+#include "sgk_batch.h"
 #include "sgk_params.h"
 
 namespace sgk {
@@ -69,19 +70,103 @@ struct sgk_ctx {
     std::string err;
 };
 
+// A batch of replicas advanced together (sgk_batch_*): shared arrays, one range per replica.
+struct sgk_batch {
+    int nrep = 0;
+    std::vector<SimParams> sp;
+    sgk_config cfg;
+    bool mixed = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int metalLanes = 4;
+    std::vector<sgk::BatchRep> reps;  // host copy of the replica table
+    sgk::BatchRep *dReps = nullptr;
+    int2 *atomTab = nullptr, *forceTab = nullptr, *metalTab = nullptr;
+    int nAtomBlocks = 0, nForceBlocks = 0, nMetalBlocks = 0;
+    long long nAtoms = 0, nMetal = 0, nCarbon = 0, nCells = 0;
+    void *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;
+    int *cellOf[2] = {nullptr, nullptr}, *cnt[2] = {nullptr, nullptr}, *slots[2] = {nullptr, nullptr};
+    int *slotOf = nullptr;
+    float4 *cellPos = nullptr;
+    int *nlM = nullptr, *nnM = nullptr, *nlj = nullptr, *ncl = nullptr;
+    unsigned int *lj = nullptr, *cl = nullptr;
+    uint4 *bondC = nullptr, *bondA = nullptr;
+    void *partials = nullptr;
+    sgk::DevState *state = nullptr;
+    sgk_record *records = nullptr;
+    unsigned int *hist = nullptr;
+    sgk::DevState *hostState = nullptr;  // pinned mirror, nrep entries
+    long long bytes = 0;
+    int hostStep = 0;
+    double lastRunMs = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<std::string, sgk::GraphEntry> graphs;
+    std::string err;
+    std::vector<void *> allocs;
+
+    sgk::BatchArgs args() const {
+        sgk::BatchArgs a;
+        a.reps = dReps;
+        a.atomTab = atomTab;
+        a.forceTab = forceTab;
+        a.metalTab = metalTab;
+        a.pos = pos;
+        a.vel = vel;
+        a.acc = acc;
+        a.rho = rho;
+        a.embF = embF;
+        a.embDF = embDF;
+        a.fric = fric;
+        for (int k = 0; k < 2; ++k) {
+            a.cellOf[k] = cellOf[k];
+            a.cnt[k] = cnt[k];
+            a.slots[k] = slots[k];
+        }
+        a.slotOf = slotOf;
+        a.cellPos = cellPos;
+        a.nlM = nlM;
+        a.nnM = nnM;
+        a.nlj = nlj;
+        a.ncl = ncl;
+        a.lj = lj;
+        a.cl = cl;
+        a.bondC = bondC;
+        a.bondA = bondA;
+        a.partials = partials;
+        a.state = state;
+        a.records = records;
+        a.recCap = cfg.records_capacity;
+        a.hist = hist;
+        return a;
+    }
+};
+
 namespace sgk {
+
+// Device parameters of one system (sgk_api.hip).
+void make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &p);
 
 // sgk_nebr.hip
 void launch_bin(sgk_ctx *c, bool force);
+void launch_batch_sortcells(sgk_batch *b);
+void launch_batch_nl(sgk_batch *b);
+void launch_batch_rdf(sgk_batch *b);
 void launch_sortcells(sgk_ctx *c, bool force);
 void launch_nl(sgk_ctx *c, bool force);
 void launch_commit_rebuild(sgk_ctx *c);
 void launch_rdf(sgk_ctx *c);
 // sgk_force.hip
 void launch_rho(sgk_ctx *c);
+void launch_batch_rho(sgk_batch *b);
+void launch_batch_force(sgk_batch *b);
 void launch_force(sgk_ctx *c, bool integrate, bool finalize);
 // sgk_step.hip
 void launch_lf1(sgk_ctx *c, bool bin);
+void launch_batch_lf1(sgk_batch *b);
+void launch_batch_thermostat(sgk_batch *b);
+void launch_batch_shear(sgk_batch *b);
+void launch_batch_reset_records(sgk_batch *b);
 void launch_lf2(sgk_ctx *c);
 void launch_props(sgk_ctx *c);
 void launch_finalize(sgk_ctx *c, int mode);          // 0 step, 1 evaluate only
@@ -95,5 +180,6 @@ void profile_end(sgk_ctx *c, const char *name);
 
 // Partial-record size for the context precision.
 size_t partial_bytes(const sgk_ctx *c);
+size_t partial_bytes_for(bool mixed);
 
 }  // namespace sgk

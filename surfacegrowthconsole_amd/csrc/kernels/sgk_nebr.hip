@@ -17,6 +17,7 @@
 // positions.  (The graphene sheet lies exactly on a cell boundary in z by construction, so the
 // cell of every carbon atom - and the list order the carbon force loop depends on - is decided by
 // that rounding.)
+#include "sgk_batch.h"
 #include "sgk_cells.h"
 #include "sgk_device.h"
 #include "sgk_internal.h"
@@ -87,12 +88,12 @@ __global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
 // atoms' float positions in slot order; clear the counts of the outgoing buffer.  One thread per
 // atom; the owner of a cell is the atom holding slot 0, and the owner's wave sorts the cell.
 template <bool M>
-__global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                                   CellBufs bufs, DevState *st, int force) {
+__device__ __forceinline__ void sortcells_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                               const CellBufs &bufs, DevState *st, int force) {
     if (!force && !st->nebrNow) return;
     const int b = st->cellbuf ^ 1, old = st->cellbuf;
     const int lane = threadIdx.x & 63;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blk * blockDim.x + threadIdx.x;
     const bool in = n < p.nMol;  // every lane stays for the wave-wide sort
     int cn = -1;
     if (in) {
@@ -127,6 +128,19 @@ __global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename P
     }
 }
 
+template <bool M>
+__global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                                   CellBufs bufs, DevState *st, int force) {
+    sortcells_body<M>(p, blockIdx.x, pos, bufs, st, force);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void kb_sortcells(BatchArgs a) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    sortcells_body<M>(B.p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0);
+}
+
 // BuildNebrListK (SurfaceGrowth.cu:612-711), one thread per atom over cells whose slots are sorted
 // ascending (k_sortcells), neighbours appended in the reference order: 27 offset cells in table
 // order, atoms of a cell ascending.  Slots are read 8 at a time with vector loads and their
@@ -136,14 +150,13 @@ __global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename P
 //                cl[ic*clCap + q] = k<<8 | j if r^2 < closeR2 (bond and angle candidates)
 //   keepFull:    fullNBL[j*nMol + i] = k, fullNN[i]      (reference layout)
 template <bool M>
-__global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                            CellBufs bufs, DevState *st, int force,
-                                            int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
-                                            unsigned *__restrict__ lj, int *__restrict__ nlj,
-                                            unsigned *__restrict__ cl, int *__restrict__ ncl,
-                                            int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+__device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                        const CellBufs &bufs, DevState *st, int force, int fromNext,
+                                        int *__restrict__ nlM, int *__restrict__ nnM, unsigned *__restrict__ lj,
+                                        int *__restrict__ nlj, unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                        int *__restrict__ fullNN, int *__restrict__ fullNBL) {
     if (!force && !st->nebrNow) return;
-    const int i = ((p.xcdMap & 1) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= p.nMol) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
     const int *cnt = bufs.cnt[b];
@@ -233,6 +246,28 @@ __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>:
     if (err) atomicOr(&st->errors, err);
 }
 
+template <bool M>
+__global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                            CellBufs bufs, DevState *st, int force,
+                                            int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
+                                            unsigned *__restrict__ lj, int *__restrict__ nlj,
+                                            unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                            int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    const int blk = (p.xcdMap & 1) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    nl_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
+}
+
+// Batched: the step path (lists built from the new cell buffer before the commit).
+template <bool M>
+__global__ __launch_bounds__(256) void kb_nl(BatchArgs a) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const DevParams &p = B.p;
+    nl_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
+               a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff,
+               a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
+}
+
 // Make the freshly built buffer current (API path; the step path does this in its reduction).
 __global__ void k_commit(DevState *st) {
     st->cellbuf ^= 1;
@@ -243,12 +278,12 @@ __global__ void k_commit(DevState *st) {
 // cells, r^2 < rangeRdf^2, bin = int(sqrt(r^2) * intervalRdf).  One thread per metal atom.
 // Histogram in LDS, merged with one atomic per bin and block; counts are order independent.
 template <bool M>
-__global__ __launch_bounds__(256) void k_rdf(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                             CellBufs bufs, DevState *st, unsigned *__restrict__ hist) {
-    extern __shared__ unsigned sh[];
+__device__ __forceinline__ void rdf_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                         const CellBufs &bufs, const DevState *st, unsigned *__restrict__ hist,
+                                         unsigned *sh, int *errors) {
     for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     const int c = i < p.nMolMe ? bufs.cellOf[st->cellbuf][i] : -1;
     int edge = 0;
     if (c >= 0) {
@@ -279,10 +314,26 @@ __global__ __launch_bounds__(256) void k_rdf(DevParams p, const typename Prec<M>
             }
         }
     }
-    if (edge) atomicOr(&st->errors, SGK_FLAG_RDF_EDGE);
+    if (edge) atomicOr(errors, SGK_FLAG_RDF_EDGE);
     __syncthreads();
     for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x)
         if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_rdf(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                             CellBufs bufs, DevState *st, unsigned *__restrict__ hist) {
+    extern __shared__ unsigned sh[];
+    rdf_body<M>(p, blockIdx.x, pos, bufs, st, hist, sh, &st->errors);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void kb_rdf(BatchArgs a) {
+    extern __shared__ unsigned sh[];
+    const int2 e = a.metalTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    rdf_body<M>(B.p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x,
+                a.hist + (size_t)e.x * B.p.sizeHistRdf, sh, &a.state[e.x].errors);
 }
 
 // ---- launchers ---------------------------------------------------------------------------------
@@ -328,6 +379,29 @@ void launch_nl(sgk_ctx *c, bool force) {
                            c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
                            c->fullNN, c->fullNBL);
     profile_end(c, "nl");
+}
+
+void launch_batch_sortcells(sgk_batch *b) {
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_sortcells<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_sortcells<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+}
+
+void launch_batch_nl(sgk_batch *b) {
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_nl<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_nl<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+}
+
+void launch_batch_rdf(sgk_batch *b) {
+    if (b->nMetalBlocks == 0) return;
+    const size_t shm = sizeof(unsigned) * (size_t)b->sp[0].sizeHistRdf;
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_rdf<true>, dim3(b->nMetalBlocks), dim3(256), shm, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_rdf<false>, dim3(b->nMetalBlocks), dim3(256), shm, b->stream, b->args());
 }
 
 void launch_commit_rebuild(sgk_ctx *c) {

@@ -2,6 +2,7 @@
 // device-resident run state.  Plain structs usable from host and device code.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace sgk {
@@ -32,6 +33,18 @@ struct DevParams {
     int rebuildEvery;
     float dispBound;           // max dispHi for which the carbon close lists are exact
     int xcdMap, binGroup;      // launch-shape options (sgk_config.xcd_map, bin_group)
+};
+
+// Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).
+// cellOf[b][n] is atom n's cell in buffer b (-1: not binned); slotOf[n] its slot in the buffer
+// being built; cellPos[c*cap + s] the float position of the atom in slot s of cell c after the sort
+// (built with the buffer, read by the neighbour-list pass).
+struct CellBufs {
+    int *cnt[2];
+    int *slots[2];
+    int *cellOf[2];
+    int *slotOf;
+    float4 *cellPos;
 };
 
 // Per-block partial reductions written by the force kernels (or the props kernel).

@@ -5,6 +5,7 @@
 // one Partial per block and k_finalize (one workgroup) reduces them in a fixed order, evaluates the
 // same host arithmetic on the device, updates dispHi / nebrNow / shear and appends an sgk_record,
 // so consecutive steps run without any host round trip.
+#include "sgk_batch.h"
 #include "sgk_cells.h"
 #include "sgk_device.h"
 #include "sgk_internal.h"
@@ -16,13 +17,13 @@ namespace sgk {
 // when the device flag asks for a rebuild, with the cell binning of the new position
 // (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
 template <bool M>
-__global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
-                                             typename Prec<M>::R *__restrict__ vel,
-                                             const typename Prec<M>::R *__restrict__ acc,
-                                             CellBufs bufs, DevState *st, int doBin) {
+__device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos,
+                                         typename Prec<M>::R *__restrict__ vel,
+                                         const typename Prec<M>::R *__restrict__ acc, const CellBufs &bufs,
+                                         DevState *st, int doBin) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blk * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
     auto r = pos[n];
     bool move = true;
@@ -50,6 +51,25 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
     else if (r.z < -R(0.5f) * Lz) r.z += Lz;
     pos[n] = r;
     if (doBin && st->nebrNow) bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
+                                             typename Prec<M>::R *__restrict__ vel,
+                                             const typename Prec<M>::R *__restrict__ acc,
+                                             CellBufs bufs, DevState *st, int doBin) {
+    lf1_body<M>(p, blockIdx.x, pos, vel, acc, bufs, st, doBin);
+}
+
+// Batched: one workgroup of one replica per table entry.
+template <bool M>
+__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a) {
+    using P4 = typename Prec<M>::P4;
+    using R = typename Prec<M>::R;
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
+                rep_cells(a, B), a.state + e.x, 1);
 }
 
 // LeapfrogStepK part 2 alone (the step loop fuses it into the force kernels).
@@ -119,12 +139,12 @@ __global__ __launch_bounds__(256) void k_finalize(DevParams p, const Partial<typ
 
 // ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485), kinetic energy of all atoms.
 template <bool M>
-__global__ __launch_bounds__(256) void k_thermostat(DevParams p, typename Prec<M>::R *__restrict__ vel,
-                                                    const DevState *st, int stepOverride, double vvOverride,
-                                                    int useOverride, int metalAll) {
+__device__ __forceinline__ void thermostat_body(const DevParams &p, int blk, typename Prec<M>::R *__restrict__ vel,
+                                                const DevState *st, int stepOverride, double vvOverride,
+                                                int useOverride, int metalAll) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blk * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
     const int step = useOverride ? stepOverride : st->step;
     const A vv = (A)(useOverride ? vvOverride : st->vvSum);
@@ -144,14 +164,27 @@ __global__ __launch_bounds__(256) void k_thermostat(DevParams p, typename Prec<M
     vel[2 * p.nMol + n] = (R)(b * (A)vel[2 * p.nMol + n]);
 }
 
+template <bool M>
+__global__ __launch_bounds__(256) void k_thermostat(DevParams p, typename Prec<M>::R *__restrict__ vel,
+                                                    const DevState *st, int stepOverride, double vvOverride,
+                                                    int useOverride, int metalAll) {
+    thermostat_body<M>(p, blockIdx.x, vel, st, stepOverride, vvOverride, useOverride, metalAll);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void kb_thermostat(BatchArgs a, int metalAll) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    thermostat_body<M>(B.p, e.y, (typename Prec<M>::R *)a.vel + 3 * B.atomOff, a.state + e.x, 0, 0.0, 0, metalAll);
+}
+
 // ApplyShearK (SurfaceGrowth.cu:517-534): metal atoms left of the centre of mass get the shear
 // acceleration; the count goes into the step's record.
 template <bool M>
-__global__ __launch_bounds__(256) void k_shear(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
-                                               typename Prec<M>::R *__restrict__ acc, const DevState *st,
-                                               sgk_record *records,
-                                               int recCap) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shear_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                           typename Prec<M>::R *__restrict__ acc, const DevState *st,
+                                           sgk_record *records, int recCap) {
+    const int n = blk * blockDim.x + threadIdx.x;
     int hit = 0;
     if (n < p.nMolMe) {
         const float dr = (float)st->comX - (float)pos[n].x;
@@ -165,12 +198,35 @@ __global__ __launch_bounds__(256) void k_shear(DevParams p, const typename Prec<
     if (threadIdx.x == 0 && cnt && slot >= 0 && slot < recCap) atomicAdd(&records[slot].shearCount, (double)cnt);
 }
 
+template <bool M>
+__global__ __launch_bounds__(256) void k_shear(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                               typename Prec<M>::R *__restrict__ acc, const DevState *st,
+                                               sgk_record *records, int recCap) {
+    shear_body<M>(p, blockIdx.x, pos, acc, st, records, recCap);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void kb_shear(BatchArgs a) {
+    const int2 e = a.metalTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    if (B.p.deltaF == 0.f) return;  // uniform per workgroup; this replica's shear stays 0
+    shear_body<M>(B.p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff,
+                  (typename Prec<M>::R *)a.acc + 3 * B.atomOff, a.state + e.x, a.records + (size_t)e.x * a.recCap,
+                  a.recCap);
+}
+
 __global__ void k_reset_records(DevState *st) { st->rec = 0; }
+
+__global__ void kb_reset_records(DevState *st, int nrep) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrep) st[r].rec = 0;
+}
 
 // ---- launchers ---------------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-size_t partial_bytes(const sgk_ctx *c) { return c->mixed ? sizeof(Partial<double>) : sizeof(Partial<float>); }
+size_t partial_bytes_for(bool mixed) { return mixed ? sizeof(Partial<double>) : sizeof(Partial<float>); }
+size_t partial_bytes(const sgk_ctx *c) { return partial_bytes_for(c->mixed); }
 
 void launch_lf1(sgk_ctx *c, bool bin) {
     profile_begin(c);
@@ -249,5 +305,33 @@ void launch_shear(sgk_ctx *c) {
 }
 
 void launch_reset_records(sgk_ctx *c) { hipLaunchKernelGGL(k_reset_records, dim3(1), dim3(1), 0, c->stream, c->state); }
+
+// ---- batched launchers -----------------------------------------------------------------------
+void launch_batch_lf1(sgk_batch *b) {
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+}
+
+void launch_batch_thermostat(sgk_batch *b) {
+    const int metalAll = b->cfg.metal_thermostat_all;
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_thermostat<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), metalAll);
+    else
+        hipLaunchKernelGGL(kb_thermostat<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), metalAll);
+}
+
+void launch_batch_shear(sgk_batch *b) {
+    if (b->nMetalBlocks == 0) return;
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_shear<true>, dim3(b->nMetalBlocks), dim3(256), 0, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_shear<false>, dim3(b->nMetalBlocks), dim3(256), 0, b->stream, b->args());
+}
+
+void launch_batch_reset_records(sgk_batch *b) {
+    hipLaunchKernelGGL(kb_reset_records, dim3((b->nrep + 255) / 256), dim3(256), 0, b->stream, b->state, b->nrep);
+}
 
 }  // namespace sgk

@@ -257,6 +257,97 @@ class Engine:
         return self.lib.sgk_device_bytes(self.h)
 
 
+class Batch:
+    """Many replicas advanced together (sgk_batch): one launch per stage for all of them.  Replicas
+    share the potentials, time step and schedules; geometry, atom counts and temperature differ."""
+
+    def __init__(self, params_list, precision="mixed", rebuild_every=0, metal_lanes=0, records_capacity=1024,
+                 use_graphs=True, **cfg):
+        self.lib = N.lib()
+        self.params = list(params_list)
+        c = N.SgkConfig()
+        self.lib.sgk_default_config(C.byref(c))
+        c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        c.rebuild_every = rebuild_every
+        c.metal_lanes = metal_lanes
+        c.records_capacity = records_capacity
+        c.use_graphs = int(use_graphs)
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        arr = (N.SimParams * len(self.params))(*self.params)
+        h = C.c_void_p()
+        rc = self.lib.sgk_batch_create(len(self.params), C.cast(arr, C.c_void_p), C.byref(c), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"sgk_batch_create failed ({rc}): {self.lib.sgk_batch_last_error(None).decode()}")
+        self.h = h
+        self.nrep = len(self.params)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.sgk_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sgk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def metal_lanes(self):
+        return self.lib.sgk_batch_metal_lanes(self.h)
+
+    @property
+    def atoms(self):
+        return self.lib.sgk_batch_atoms(self.h)
+
+    def upload(self, rep, r, v=None, a=None):
+        n = self.params[rep].nMol
+        r4 = np.zeros((n, 4))
+        r = np.asarray(r, np.float64)
+        r4[:, : r.shape[1]] = r
+        args = [np.ascontiguousarray(x, np.float64) if x is not None else None for x in (r4, v, a)]
+        self._check(self.lib.sgk_batch_upload_f64(self.h, rep, *[_ptr(x) if x is not None else None for x in args]),
+                    "sgk_batch_upload_f64")
+
+    def download(self, rep):
+        n = self.params[rep].nMol
+        r, v, a = np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._check(self.lib.sgk_batch_download_f64(self.h, rep, _ptr(r), _ptr(v), _ptr(a)), "sgk_batch_download_f64")
+        return r, v, a
+
+    def set_run_state(self, rep, step=0, nebr_now=1, disp_hi=0.0):
+        self._check(self.lib.sgk_batch_set_run_state(self.h, rep, step, nebr_now, disp_hi), "sgk_batch_set_run_state")
+
+    def run(self, nsteps, rdf_every=0, records=True):
+        """nsteps steps of every replica; returns one list of record dicts per replica."""
+        if not records:
+            self._check(self.lib.sgk_batch_run(self.h, nsteps, rdf_every, None), "sgk_batch_run")
+            return None
+        buf = (N.SgkRecord * (nsteps * self.nrep))()
+        self._check(self.lib.sgk_batch_run(self.h, nsteps, rdf_every, buf), "sgk_batch_run")
+        return [[record_dict(buf[r * nsteps + k]) for k in range(nsteps)] for r in range(self.nrep)]
+
+    def last_run_ms(self):
+        return self.lib.sgk_batch_last_run_ms(self.h)
+
+    def rdf(self, rep):
+        h = np.zeros(self.params[rep].sizeHistRdf, np.uint32)
+        self._check(self.lib.sgk_batch_rdf_get(self.h, rep, _ptr(h)), "sgk_batch_rdf_get")
+        return h
+
+
 def record_dict(r):
     d = {}
     for name, _ in N.SgkRecord._fields_:
