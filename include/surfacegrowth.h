@@ -233,6 +233,15 @@ int sgWriteBackup(const char *path, const SimParams *p, const sg_float4 *hr, con
 int sgReadBackup(const char *path, SimParams *p, sg_float4 *hr, sg_float3 *hv, sg_float3 *ha,
                  int capacity);
 
+/* Centre-of-mass diffusion buffers that DoComputationsW keeps (EvalDiffusion, AccumDiffusion,
+ * PrintDiffusion: SurfaceGrowth.cu:2865-2952), callable directly.  Each call takes one sample of
+ * the metal centre of mass; when an average of limitDiffuseAv buffers completes, a "diffusion"
+ * table is appended to `file` (may be NULL).  Returns the number of tables completed so far. */
+typedef struct sgDiffusion sgDiffusion;
+sgDiffusion *sgDiffusionCreate(const SimParams *p);
+int sgDiffusionEval(sgDiffusion *d, SimParams *p, const sg_float3 *com, FILE *file);
+void sgDiffusionDestroy(sgDiffusion *d);
+
 /* Library version string. */
 const char *sgVersion(void);
 /* sizeof of the ABI structs in the order SimParams, SgOptions, SgInput, TBuf, MATERIAL (bindings

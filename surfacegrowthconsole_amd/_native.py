@@ -139,6 +139,7 @@ EXPORTED_SYMBOLS = [
     "sgDefaultOptions", "sgSetOptions", "sgGetOptions", "sgDefaultInput", "sgReadInputFile",
     "sgSetParams", "sgInitRand", "sgRandR", "sgVRand", "sgInitVels", "sgOpenResultFile",
     "sgWriteBackup", "sgReadBackup", "sgVersion", "sgStructSizes",
+    "sgDiffusionCreate", "sgDiffusionEval", "sgDiffusionDestroy",
     "sgk_default_config", "sgk_create", "sgk_destroy", "sgk_last_error", "sgk_device_count",
     "sgk_upload", "sgk_download", "sgk_upload_f64", "sgk_download_f64", "sgk_set_run_state",
     "sgk_bin", "sgk_get_cells", "sgk_build_nebr", "sgk_get_nebr", "sgk_get_carbon_lists",
@@ -163,6 +164,10 @@ def _proto(lib):
         "sgGetOptions": (None, [P(SgOptions)]),
         "sgDefaultInput": (None, [P(SgInput)]),
         "sgReadInputFile": (_i, [C.c_char_p, P(SgInput), P(SgOptions)]),
+        "sgOpenResultFile": (vp, [P(SimParams), _f, C.c_char_p]),
+        "sgDiffusionCreate": (vp, [P(SimParams)]),
+        "sgDiffusionEval": (_i, [vp, P(SimParams), P(Float3), vp]),
+        "sgDiffusionDestroy": (None, [vp]),
         "sgSetParams": (_i, [P(SgInput), P(SimParams), C.c_longlong]),
         "sgInitRand": (None, [_i, P(SimParams)]),
         "sgRandR": (_f, [P(SimParams)]),

@@ -195,3 +195,25 @@ void Diffusion::eval(SimParams &p, sg_float3 com, FILE *file) {
 }
 
 }  // namespace sg
+
+struct sgDiffusion {
+    sg::Diffusion d;
+    explicit sgDiffusion(const SimParams &p) : d(p) {}
+};
+
+extern "C" {
+
+sgDiffusion *sgDiffusionCreate(const SimParams *p) {
+    if (!p || p->nValDiffuse <= 0 || p->nBuffDiffuse <= 0) return nullptr;
+    return new sgDiffusion(*p);
+}
+
+int sgDiffusionEval(sgDiffusion *d, SimParams *p, const sg_float3 *com, FILE *file) {
+    if (!d || !p || !com) return -1;
+    d->d.eval(*p, *com, file);
+    return d->d.tables_printed();
+}
+
+void sgDiffusionDestroy(sgDiffusion *d) { delete d; }
+
+}  // extern "C"
