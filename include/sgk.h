@@ -87,6 +87,8 @@ typedef struct sgk_record {
     double dispHi;        /* after this step */
     double shear;         /* current shear increment state (regime 2) */
     double shearCount;    /* atoms sheared in this step */
+    int nMolDeposited;    /* metal atoms deposited so far (regime 1) */
+    int pad;
 } sgk_record;
 
 void sgk_default_config(sgk_config *cfg);
@@ -167,6 +169,10 @@ int sgk_kernel_stats(sgk_ctx *ctx, sgk_kernel_stat *stats, int max);
 long long sgk_device_bytes(sgk_ctx *ctx);
 /* sizeof of sgk_config, sgk_record, sgk_kernel_stat (for bindings); returns the number written. */
 int sgk_struct_sizes(int *out, int max);
+/* Metal atoms already deposited (regime 1; the reference's hparams->nMolDeposited).  Creation
+ * takes it from SimParams. */
+int sgk_set_deposited(sgk_ctx *ctx, int nMolDeposited);
+
 /* Lanes per metal atom in use (chosen at creation, or by sgk_autotune). */
 int sgk_metal_lanes(const sgk_ctx *ctx);
 

@@ -47,6 +47,9 @@ typedef struct orc_params {
     int sizeHistRdf;
     /* shear */
     float deltaF;
+    /* deposition (regime 1) */
+    float velMagDepos;
+    int stepDeposit, nMolToDeposit;
 } orc_params;
 
 /* Reductions of one state with the reference semantics (.cu:1101-1891, 2307-2395). */
@@ -59,7 +62,7 @@ typedef struct orc_props {
 
 /* Per-step record of orc_run (same meaning as sgk_record). */
 typedef struct orc_record {
-    int step, rebuilt, nebrNow, pad;
+    int step, rebuilt, nebrNow, nMolDeposited;
     double kinEnergy, potEnergy, totEnergy;
     double vSum[3], vvSum, vvMax, uSum;
     double comSum[3], cmVelSum[3], frict;
@@ -119,12 +122,15 @@ int orc_diffusion_sample_f32(orc_diffusion *d, const float *com, float *out);
 int orc_diffusion_sample_f64(orc_diffusion *d, const double *com, double *out);
 
 /* Whole loop of DoComputationsW (.cu:2141-2489) without output files: nsteps steps from
- * stepCount0 with the given rebuild state.  rebuildEvery>0 replaces the adaptive rule.
+ * stepCount0 with the given rebuild state.  rebuildEvery>0 adds rebuilds to the adaptive rule.
+ * nMolDeposited (regime 1; may be NULL) is the deposition counter, updated in place.
  * records[nsteps] may be NULL.  Returns 0 or an error code (1 cell overflow, 2 NL overflow). */
 int orc_run_f32(const orc_params *p, float *pos4, float *vel, float *acc, int nsteps,
-                int stepCount0, int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records);
+                int stepCount0, int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records,
+                int *nMolDeposited);
 int orc_run_f64(const orc_params *p, double *pos4, double *vel, double *acc, int nsteps,
-                int stepCount0, int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records);
+                int stepCount0, int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records,
+                int *nMolDeposited);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ class OrcParams(C.Structure):
         ("deltaT", _f), ("temperature", _f), ("kB", _f), ("gammaBerendsen", _f), ("rNebrShell", _f),
         ("stepThermostat", _i), ("stepEquil", _i), ("stepCool", _i),
         ("rangeRdf", _f), ("intervalRdf", _f), ("sizeHistRdf", _i), ("deltaF", _f),
+        ("velMagDepos", _f), ("stepDeposit", _i), ("nMolToDeposit", _i),
     ]
 
 
@@ -38,7 +39,7 @@ class OrcProps(C.Structure):
 
 class OrcRecord(C.Structure):
     _d = C.c_double
-    _fields_ = [("step", C.c_int), ("rebuilt", C.c_int), ("nebrNow", C.c_int), ("pad", C.c_int),
+    _fields_ = [("step", C.c_int), ("rebuilt", C.c_int), ("nebrNow", C.c_int), ("nMolDeposited", C.c_int),
                 ("kinEnergy", _d), ("potEnergy", _d), ("totEnergy", _d), ("vSum", _d * 3), ("vvSum", _d),
                 ("vvMax", _d), ("uSum", _d), ("comSum", _d * 3), ("cmVelSum", _d * 3), ("frict", _d),
                 ("bboxMin", _d * 3), ("bboxMax", _d * 3), ("dispHi", _d)]
@@ -72,7 +73,7 @@ def lib():
                 "orc_props": (None, [vp, vp, vp, vp, vp]),
                 "orc_rdf": (C.c_int, [vp, vp, vp, vp, vp, vp]),
                 "orc_diffusion_sample": (C.c_int, [vp, vp, vp]),
-                "orc_run": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
+                "orc_run": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
             }
             for name, (res, args) in spec.items():
                 fn = getattr(_lib, f"{name}_{sfx}")
@@ -98,7 +99,8 @@ def params_from(sp, cell_cap=64):
     for k in ("re", "fe", "rhoe", "alpha", "beta", "A", "B", "kappa", "eta", "Fe", "massMe", "rCutEam",
               "rrCutEam", "rei", "rhoei", "mu_r", "theta_0", "mu_theta", "mu_p", "z_0", "rCutC", "rrCutC",
               "sigmaLJ", "epsLJ", "rrCutLJ", "deltaT", "temperature", "kB", "gammaBerendsen", "rNebrShell",
-              "stepThermostat", "stepEquil", "stepCool", "rangeRdf", "intervalRdf", "sizeHistRdf", "deltaF"):
+              "stepThermostat", "stepEquil", "stepCool", "rangeRdf", "intervalRdf", "sizeHistRdf", "deltaF",
+              "velMagDepos", "stepDeposit", "nMolToDeposit"):
         setattr(o, k, getattr(sp, k))
     o.lambda_ = sp.lambda_
     o.Fn[:] = list(sp.Fn)
@@ -118,6 +120,7 @@ class Oracle:
         self.ncell = sp.cells.x * sp.cells.y * sp.cells.z
         self.cap = cell_cap
         self.nbmax = sp.iNebrMax
+        self.deposited = sp.nMolDeposited
 
     def f(self, name):
         return getattr(self.l, f"{name}_{self.sfx}")
@@ -197,13 +200,15 @@ class Oracle:
                                  _p(np.ascontiguousarray(slots, np.int32)), _p(hist))
         return hist, edge
 
-    def run(self, r, v, a, nsteps, step0=0, nebr_now=1, disp_hi=0.0, rebuild_every=0):
+    def run(self, r, v, a, nsteps, step0=0, nebr_now=1, disp_hi=0.0, rebuild_every=0, deposited=None):
+        """deposited: regime-1 deposition counter at the start (default: the SimParams value)."""
         r, v, a = self.arr(r).copy(), self.arr(v).copy(), self.arr(a).copy()
         recs = (OrcRecord * nsteps)()
         nb = C.c_int(nebr_now)
         dh = C.c_double(disp_hi)
+        dep = C.c_int(self.deposited if deposited is None else deposited)
         err = self.f("orc_run")(C.byref(self.p), _p(r), _p(v), _p(a), nsteps, step0, C.byref(nb),
-                                C.byref(dh), rebuild_every, recs)
+                                C.byref(dh), rebuild_every, recs, C.byref(dep))
         out = [{k: (list(getattr(x, k)) if hasattr(getattr(x, k), "__len__") else getattr(x, k))
                 for k, _ in OrcRecord._fields_} for x in recs]
         return err, r, v, a, out

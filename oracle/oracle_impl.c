@@ -634,9 +634,26 @@ int FN(orc_diffusion_sample)(orc_diffusion *d, const REAL *com, REAL *out) {
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* InsertAtomsK (.cu:488-514): the group [dep, dep + toDeposit) enters at z = 0.49 Lz moving down
+ * at velMagDepos; metal atoms not yet due are parked above the box at rest. */
+static void FN(orc_insert)(const orc_params *p, REAL *pos4, REAL *vel, int dep, int toDeposit) {
+    const float Lz = p->region[2];
+    const int interval = dep + toDeposit;
+    for (int n = 0; n < p->nMolMe; ++n) {
+        if (n >= dep && n < interval) {
+            pos4[4 * n + 2] = (REAL)(0.49f * Lz);
+            vel[3 * n + 2] = (REAL)(-p->velMagDepos);
+        }
+        if (n >= interval) {
+            pos4[4 * n + 2] = (REAL)(1.5f * Lz + 0.5f * (float)(n + 1) * Lz);
+            vel[3 * n] = vel[3 * n + 1] = vel[3 * n + 2] = 0;
+        }
+    }
+}
+
 /* The step loop of DoComputationsW (.cu:2141-2489) with device work restated serially. */
 int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nsteps, int stepCount0,
-                int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records) {
+                int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records, int *nMolDeposited) {
     const int nMol = p->nMol, nMe = p->nMolMe;
     const int ncell = p->cells[0] * p->cells[1] * p->cells[2];
     int *cellOf = (int *)malloc(sizeof(int) * (size_t)nMol);
@@ -652,6 +669,16 @@ int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nstep
         const int step = stepCount0 + s + 1;
         orc_record rec;
         memset(&rec, 0, sizeof(rec));
+        if (p->iRegime == 1 && nMe && nMolDeposited) { /* deposition schedule (.cu:2148-2168) */
+            if (step <= p->stepEquil) {
+                FN(orc_insert)(p, pos4, vel, *nMolDeposited, 0);
+            } else if (*nMolDeposited < nMe) {
+                int to = 0;
+                if (p->stepDeposit > 0 && (step - p->stepEquil) % p->stepDeposit == 0) to = p->nMolToDeposit;
+                FN(orc_insert)(p, pos4, vel, *nMolDeposited, to);
+                *nMolDeposited += to;
+            }
+        }
         FN(orc_leapfrog)(p, 1, pos4, vel, acc);
         if (*nebrNow || !haveList) {
             *nebrNow = 0;
@@ -698,6 +725,7 @@ int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nstep
             rec.uSum = pr.uSum;
             rec.frict = pr.frict;
             rec.dispHi = dh;
+            rec.nMolDeposited = nMolDeposited ? *nMolDeposited : nMe;
             records[s] = rec;
         }
     }

@@ -120,6 +120,7 @@ class SgkRecord(C.Structure):
         ("kinEnergy", _d), ("potEnergy", _d), ("totEnergy", _d), ("vSum", _d * 3), ("vvSum", _d),
         ("vvMax", _d), ("uSum", _d), ("comSum", _d * 3), ("cmVelSum", _d * 3), ("frict", _d),
         ("bboxMin", _d * 3), ("bboxMax", _d * 3), ("dispHi", _d), ("shear", _d), ("shearCount", _d),
+        ("nMolDeposited", _i), ("pad", _i),
     ]
 
 
@@ -146,7 +147,7 @@ EXPORTED_SYMBOLS = [
     "sgk_compute_rho", "sgk_compute_forces", "sgk_get_rho", "sgk_get_forces", "sgk_leapfrog",
     "sgk_thermostat", "sgk_eval_props", "sgk_rdf_clear", "sgk_rdf_accumulate", "sgk_rdf_get",
     "sgk_run", "sgk_last_run_ms", "sgk_set_profiling", "sgk_kernel_stats", "sgk_device_bytes",
-    "sgk_struct_sizes", "sgk_metal_lanes", "sgk_autotune",
+    "sgk_struct_sizes", "sgk_metal_lanes", "sgk_autotune", "sgk_set_deposited",
     "sgk_batch_create", "sgk_batch_destroy", "sgk_batch_last_error", "sgk_batch_upload_f64",
     "sgk_batch_download_f64", "sgk_batch_set_run_state", "sgk_batch_run", "sgk_batch_last_run_ms",
     "sgk_batch_rdf_get", "sgk_batch_rdf_clear", "sgk_batch_device_bytes", "sgk_batch_metal_lanes",
@@ -213,6 +214,7 @@ def _proto(lib):
         "sgk_struct_sizes": (_i, [ip, _i]),
         "sgk_metal_lanes": (_i, [vp]),
         "sgk_autotune": (_i, [vp, _i, vp, _i, vp]),
+        "sgk_set_deposited": (_i, [vp, _i]),
         "sgk_batch_create": (_i, [_i, vp, P(SgkConfig), P(vp)]),
         "sgk_batch_destroy": (_i, [vp]),
         "sgk_batch_last_error": (C.c_char_p, [vp]),

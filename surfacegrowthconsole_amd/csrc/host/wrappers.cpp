@@ -87,9 +87,6 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
                       char *szPdbPath) {
     SimParams &p = *hparams;
     const SgOptions &o = sg::options();
-    if (p.iRegime == 1)
-        return fail(szPdbPath, "Problems with insertion of atoms! Exception: ",
-                    "regime 1 (surface growth by deposition) is not implemented by this engine");
     std::unique_ptr<sg::Diffusion> diff;
     FILE *fileDiffuse = nullptr;
     if (p.iRegime == 2) {
@@ -117,7 +114,8 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
         return fail(szPdbPath, what, msg.c_str());
     };
     if (sgk_upload(ctx, hr, hv, ha) != SGK_OK) return bail("Problems with memory allocation! Exception: ");
-    if (sgk_set_run_state(ctx, p.stepCount, p.nebrNow, p.dispHi) != SGK_OK)
+    if (sgk_set_run_state(ctx, p.stepCount, p.nebrNow, p.dispHi) != SGK_OK ||
+        sgk_set_deposited(ctx, p.nMolDeposited) != SGK_OK)
         return bail("Problems with memory allocation! Exception: ");
     if (p.bPdb != 0) sg::create_pdb_file(szPdbPath, p, hr);
 
@@ -141,6 +139,7 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
         for (int k = 0; k < n; ++k) {
             const sgk_record &r = recs[k];
             p.stepCount = r.step;
+            p.nMolDeposited = r.nMolDeposited;
             if (p.nMolMe != 0) {
                 p.centerOfMass.val = (float)r.comSum[0] / p.nMolMe;
                 p.cmVel.val = (float)r.cmVelSum[0] / p.nMolMe;

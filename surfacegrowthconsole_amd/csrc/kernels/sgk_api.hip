@@ -196,6 +196,9 @@ void sgk::make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &
     p.intervalRdf = s.intervalRdf;
     p.sizeHistRdf = s.sizeHistRdf;
     p.deltaF = s.deltaF;
+    p.velMagDepos = s.velMagDepos;
+    p.stepDeposit = s.stepDeposit;
+    p.nMolToDeposit = s.nMolToDeposit;
     p.cellCap = cfg.cell_capacity;
     p.nlCap = cfg.nl_capacity;
     p.ljCap = cfg.lj_capacity;
@@ -344,6 +347,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     std::memset(c->hostState, 0, sizeof(DevState));
     c->hostState->nebrNow = 1;
     c->hostState->step = sp->stepCount;
+    c->hostState->nMolDeposited = sp->nMolDeposited;
     if (hipMemcpyAsync(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         g_create_error = "state initialisation failed";
@@ -754,6 +758,7 @@ static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
     sgk::launch_reset_records(c);
     for (char ch : flags) {
         const int b = ch - 'a';
+        sgk::launch_insert(c);  // regime 1 only
         sgk::launch_lf1(c, true);
         sgk::launch_sortcells(c, false);
         sgk::launch_nl(c, false);
@@ -770,7 +775,6 @@ int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
     if (nsteps <= 0) return SGK_OK;
     if (nsteps > c->cfg.records_capacity)
         return fail(c, SGK_E_ARG, "nsteps exceeds records_capacity; call sgk_run in chunks");
-    if (c->sp.iRegime == 1) return fail(c, SGK_E_UNSUPPORTED, "regime 1 (deposition) is not implemented by this engine");
     const std::string flags = step_flags(c, c->hostStep, nsteps, rdfEvery);
     SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (c->cfg.use_graphs && !c->profiling) {
@@ -849,6 +853,15 @@ int sgk_struct_sizes(int *out, int max) {
     int n = 0;
     for (; n < max && n < 3; ++n) out[n] = s[n];
     return n;
+}
+
+int sgk_set_deposited(sgk_ctx *c, int nMolDeposited) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
+    c->hostState->nMolDeposited = nMolDeposited;
+    SGK_HIP(c, hipMemcpy(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice));
+    return SGK_OK;
 }
 
 static void set_metal_lanes(sgk_ctx *c, int G) {

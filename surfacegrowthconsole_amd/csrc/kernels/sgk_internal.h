@@ -163,6 +163,7 @@ void launch_batch_force(sgk_batch *b);
 void launch_force(sgk_ctx *c, bool integrate, bool finalize);
 // sgk_step.hip
 void launch_lf1(sgk_ctx *c, bool bin);
+void launch_insert(sgk_ctx *c);
 void launch_batch_lf1(sgk_batch *b);
 void launch_batch_thermostat(sgk_batch *b);
 void launch_batch_shear(sgk_batch *b);

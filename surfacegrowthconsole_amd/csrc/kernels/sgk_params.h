@@ -28,6 +28,9 @@ struct DevParams {
     int sizeHistRdf;
     // shear
     float deltaF;
+    // deposition (regime 1)
+    float velMagDepos;
+    int stepDeposit, nMolToDeposit;
     // capacities and policies
     int cellCap, nlCap, ljCap, clCap, keepFull;
     int rebuildEvery;
@@ -67,6 +70,7 @@ struct DevState {
     double comX;           // metal centre of mass x of the last step (shear input)
     double cmVelX;         // metal centre-of-mass velocity x of the last step
     unsigned int shearCount;
+    int nMolDeposited;     // metal atoms deposited so far (regime 1)
     unsigned int blocksDone;  // force-kernel blocks finished in this step (last-block reduction)
 };
 

@@ -122,6 +122,8 @@ __device__ void finalize_body(const DevParams &p, const Partial<A> *part, int nP
         rec.nebrNow = st->nebrNow;
         rec.errors = st->errors;
         rec.dispHi = st->dispHi;
+        rec.nMolDeposited = st->nMolDeposited;
+        rec.pad = 0;
         records[0] = rec;
         return;
     }
@@ -151,6 +153,12 @@ __device__ void finalize_body(const DevParams &p, const Partial<A> *part, int nP
     if (p.iRegime == 2 && nMe && step > p.stepEquil + p.stepCool) {
         if ((float)cmVelX < 0.005f) st->shear = (double)((A)st->shear + (A)p.deltaF);
     }
+    // deposition schedule (SurfaceGrowth.cu:2148-2168): the step's group counts from now on
+    if (p.iRegime == 1 && step > p.stepEquil && st->nMolDeposited < nMe && p.stepDeposit > 0 &&
+        (step - p.stepEquil) % p.stepDeposit == 0)
+        st->nMolDeposited += p.nMolToDeposit;
+    rec.nMolDeposited = st->nMolDeposited;
+    rec.pad = 0;
     rec.shear = st->shear;
     rec.nebrNow = nebrNow;
     rec.dispHi = st->dispHi;

@@ -72,6 +72,34 @@ __global__ __launch_bounds__(256) void kb_lf1(BatchArgs a) {
                 rep_cells(a, B), a.state + e.x, 1);
 }
 
+// InsertAtomsK (SurfaceGrowth.cu:488-514) with the schedule of .cu:2148-2168, for the step about to
+// run (st->step + 1): metal atoms of this step's group are placed at z = 0.49 Lz moving down at
+// velMagDepos; atoms not yet due are parked above the box at rest (outside every cell).
+template <bool M>
+__global__ __launch_bounds__(256) void k_insert(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
+                                                typename Prec<M>::R *__restrict__ vel, const DevState *st) {
+    using R = typename Prec<M>::R;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMolMe) return;
+    const int step = st->step + 1, dep = st->nMolDeposited;
+    int toDeposit = 0;
+    if (step > p.stepEquil) {
+        if (dep >= p.nMolMe) return;
+        if (p.stepDeposit > 0 && (step - p.stepEquil) % p.stepDeposit == 0) toDeposit = p.nMolToDeposit;
+    }
+    const int interval = dep + toDeposit;
+    if (n >= dep && n < interval) {
+        pos[n].z = (R)(0.49f * p.Lz);
+        vel[2 * p.nMol + n] = (R)(-p.velMagDepos);
+    }
+    if (n >= interval) {
+        pos[n].z = (R)(1.5f * p.Lz + 0.5f * (float)(n + 1) * p.Lz);
+        vel[n] = R(0);
+        vel[p.nMol + n] = R(0);
+        vel[2 * p.nMol + n] = R(0);
+    }
+}
+
 // LeapfrogStepK part 2 alone (the step loop fuses it into the force kernels).
 template <bool M>
 __global__ __launch_bounds__(256) void k_lf2(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
@@ -238,6 +266,19 @@ void launch_lf1(sgk_ctx *c, bool bin) {
         hipLaunchKernelGGL(k_lf1<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos, (float *)c->vel,
                            (const float *)c->acc, cell_bufs(c), c->state, (int)bin);
     profile_end(c, "lf1");
+}
+
+void launch_insert(sgk_ctx *c) {
+    if (c->dp.nMolMe == 0 || c->dp.iRegime != 1) return;
+    profile_begin(c);
+    const unsigned g = blocks_for(c->dp.nMolMe, 256);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_insert<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double4 *)c->pos,
+                           (double *)c->vel, c->state);
+    else
+        hipLaunchKernelGGL(k_insert<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos,
+                           (float *)c->vel, c->state);
+    profile_end(c, "insert");
 }
 
 void launch_lf2(sgk_ctx *c) {

@@ -56,7 +56,7 @@ def make_options(**kw):
 def make_system(regime=2, material=3, ux=5, uy=6, uz=0, n_metal=201, temperature=300.0, epsilon=0.023,
                 sigma=2.85, step_limit=1000, dt=0.005, step_avg=100, step_equil=0, delta_f=0.0, step_cool=0,
                 step_pdb=100000, step_rdf=50, seed=1, geometry="sphere", gap=2.5, velocities="reference",
-                **options):
+                depos_energy=None, n_to_deposit=None, step_deposit=None, **options):
     """Build a system with the reference's parameter arithmetic; extension options by keyword
     (SgOptions field names, e.g. tauPs=0.1, rangeRdfAngstrom=10.0, sizeHistRdf=200)."""
     lib = N.lib()
@@ -66,6 +66,12 @@ def make_system(regime=2, material=3, ux=5, uy=6, uz=0, n_metal=201, temperature
     inp.temperature, inp.nMolMe, inp.epsilon, inp.sigma = temperature, n_metal, epsilon, sigma
     inp.stepLimit, inp.deltaT, inp.stepAvg, inp.stepEquil = step_limit, dt, step_avg, step_equil
     inp.deltaF, inp.stepCool, inp.stepPdb, inp.stepRdf = delta_f, step_cool, step_pdb, step_rdf
+    if depos_energy is not None:  # regime 1 rows 16-18
+        inp.deposEnergy = depos_energy
+    if n_to_deposit is not None:
+        inp.nMolToDeposit = n_to_deposit
+    if step_deposit is not None:
+        inp.stepDeposit = step_deposit
     opts = make_options(seed=seed, geometry=geometry, gapAngstrom=gap, velocityRule=velocities, **options)
     lib.sgSetOptions(C.byref(opts))
     p = N.SimParams()

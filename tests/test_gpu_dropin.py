@@ -117,3 +117,19 @@ def test_console_driver_cluster_on_graphene_mixed(gpu, tmp_path):
     size_z = np.array([recs[k]["bboxMax"][2] - recs[k]["bboxMin"][2] for k in (49, 99, 149)]) * p.lengthU
     assert np.allclose(rows[:, 14], size_z, atol=2e-6)
     assert os.path.exists(tmp_path / "Diffuse.txt")
+
+
+def test_console_driver_deposition(gpu, tmp_path):
+    """Regime 1 through the console driver: the surface-growth file name (SurfaceGrowth.cpp:432-437)
+    and the PDB convention for atoms not yet deposited (z written as 2 Lz, .cu:2787-2790)."""
+    values = [1, 3, 5, 6, 0, 300, 40, 0.023, 2.85, 60, 0.005, 20, 10, 0.0, 0, 0.03, 8, 15, 30, 1000, 0, 0]
+    run_driver(tmp_path, values, ["seed=1"])
+    lines, rows = rows_of(tmp_path / "sg_x5_y6_Me40_Eq10_Dep15_TD8_Pdb30_T300Ni.txt")
+    assert rows[:, 0].tolist() == [20, 40, 60]
+    assert "deposit energy = 0.030000 eV" in lines[0]
+    s = sg.make_system(regime=1, material=3, ux=5, uy=6, n_metal=40, temperature=300.0)
+    lz2 = 1.42 * 2 * s.params.region.z
+    pdb = [l for l in open(tmp_path / "sugr_1.pdb").read().splitlines() if l.startswith("ATOM")]
+    z = np.array([float(l[47:55]) for l in pdb[:40]])
+    # at step 30 the groups of steps 25 (8 atoms) are in: the other 32 print at 2 Lz
+    assert np.sum(np.abs(z - lz2) < 0.01) == 32

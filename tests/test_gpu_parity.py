@@ -274,3 +274,30 @@ def test_autotune_keeps_results(gpu):
         b = np.array([x["totEnergy"] for x in recs_ref])
         # the XCD map regroups the block partials: FP summation order of the energy sums only
         assert np.abs(a - b).max() <= (1e-9 if prec == "mixed" else 1e-6) * np.abs(b).max()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_device_loop_deposition(gpu, prec):
+    """Regime 1: metal atoms wait above the box and enter in groups (InsertAtomsK and its schedule)
+    inside the device-resident loop, against the oracle's loop."""
+    s = sg.make_system(regime=1, material=3, ux=5, uy=6, n_metal=40, temperature=300.0, step_equil=10,
+                       depos_energy=0.03, n_to_deposit=8, step_deposit=15, dt=0.005)
+    p = s.params
+    n = 80
+    o = oracle.Oracle(p, "f64" if prec == "mixed" else "f32")
+    err, r_o, v_o, a_o, orecs = o.run(s.r, s.v, s.a, n)
+    assert err == 0
+    print(f"[parity] deposition {prec}", flush=True)
+    with sg.Engine(p, precision=prec) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.set_run_state(0, 1, 0.0)
+        recs = eng.run(n)
+        r_g, v_g, _ = eng.download()
+    assert [x["nMolDeposited"] for x in recs] == [x["nMolDeposited"] for x in orecs]
+    assert recs[-1]["nMolDeposited"] == 32  # groups of 8 at steps 25, 40, 55, 70
+    tol = 1e-6 if prec == "mixed" else 2e-4
+    for k in ("totEnergy", "potEnergy", "kinEnergy"):
+        a = np.array([x[k] for x in recs])
+        b = np.array([x[k] for x in orecs])
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), k
+    assert np.abs(r_g[:, :3] - r_o[:, :3]).max() < (1e-6 if prec == "mixed" else 1e-3)

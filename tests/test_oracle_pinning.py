@@ -495,3 +495,33 @@ def test_diffusion_rigid_translation():
     for tab in tables:
         assert tab[0] == 0.0
         assert np.allclose(tab[1:], want, rtol=1e-9)
+
+
+def test_deposition_schedule_and_parking():
+    """Regime 1 (InsertAtomsK, SurfaceGrowth.cu:488-514, schedule .cu:2148-2168): before the end of
+    equilibration every metal atom waits above the box at rest; afterwards a group of
+    nMolToDeposit atoms enters at z = 0.49 Lz with vz = -velMagDepos every stepDeposit steps."""
+    s = sg.make_system(regime=1, material=3, ux=4, uy=4, n_metal=12, temperature=300.0, step_equil=3,
+                       depos_energy=0.03, n_to_deposit=5, step_deposit=4, dt=0.005)
+    p = s.params
+    assert p.nMolDeposited == 0
+    o = oracle.Oracle(p, "f64")
+    r, v, a = s.r.astype(np.float64), s.v.astype(np.float64), s.a.astype(np.float64)
+    dep, nebr, dh = 0, 1, 0.0
+    Lz = np.float32(p.region.z)
+    for step in range(1, 16):
+        err, r, v, a, rec = o.run(r, v, a, 1, step - 1, nebr, dh, 0, dep)
+        assert err == 0
+        nebr, dh = rec[0]["nebrNow"], rec[0]["dispHi"]
+        want = dep + (5 if step > 3 and dep < 12 and (step - 3) % 4 == 0 else 0)
+        assert rec[0]["nMolDeposited"] == want, step
+        if want > dep:  # the new group entered this step at rest force (parked: no neighbours):
+            g = np.arange(dep, min(want, 12))  # z = 0.49 Lz - dt velMagDepos after the drift
+            z = float(np.float32(0.49) * Lz) - p.deltaT * float(np.float32(p.velMagDepos))
+            assert np.allclose(r[g, 2], z, atol=1e-9)
+        dep = want
+        for n in range(dep, 12):  # parked: one periodic wrap of the parked height, at rest
+            park = float(np.float32(1.5) * Lz + np.float32(0.5) * np.float32(n + 1) * Lz)
+            assert r[n, 2] == pytest.approx(park - float(Lz), rel=1e-6)
+            assert np.all(v[n] == 0.0)
+    assert dep == 15  # groups at steps 7, 11, 15 (the last group exceeds nMolMe, as in the reference)
