@@ -63,6 +63,7 @@ typedef struct sgk_config {
     int xcd_map;          /* launch shape: bit 0 neighbour list, bit 1 density/near lists, bit 2 force
                              pass - give each of the 8 XCDs a contiguous 1/8 of the atoms */
     int bin_group;        /* binning: one atomicAdd per distinct cell of a wave instead of per atom */
+    int nl_scheme;        /* neighbour list: 0 one thread per atom, 1 one wave per occupied cell */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with

@@ -246,6 +246,160 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
     if (err) atomicOr(&st->errors, err);
 }
 
+// BuildNebrListK (SurfaceGrowth.cu:612-711), one wave per occupied cell (the wave of the cell's
+// slot-0 atom).  The candidates of the 27 neighbour cells are laid over the lanes in the reference
+// order (offset table order, then ascending slot), 64 at a time; for each atom of the cell in turn
+// a ballot of the lanes that pass the distance test gives the new entries, and a prefix popcount
+// their positions - so order and positions are those of the per-atom walk.  The candidates of a
+// chunk are loaded once for all atoms of the cell, in one round trip.  Output as nl_body.
+template <bool M>
+__device__ __forceinline__ void nlw_body(const DevParams &p, int blk, const CellBufs &bufs, DevState *st,
+                                         int force, int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
+                                         unsigned *__restrict__ lj, int *__restrict__ nlj, unsigned *__restrict__ cl,
+                                         int *__restrict__ ncl, int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    if (!force && !st->nebrNow) return;
+    const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int n = blk * blockDim.x + threadIdx.x;
+    const bool in = n < p.nMol;  // every lane stays: the work below is wave-wide
+    const int cap = p.cellCap;
+    const int *cnt = bufs.cnt[b];
+    const int *slots = bufs.slots[b];
+    const float4 *cpos = bufs.cellPos;
+    int cn = -1;
+    if (in) {
+        cn = bufs.cellOf[b][n];
+        if (cn < 0) {  // not binned: empty lists
+            if (n < p.nMolMe)
+                nnM[n] = 0;
+            else {
+                nlj[n - p.nMolMe] = 0;
+                ncl[n - p.nMolMe] = 0;
+            }
+            if (p.keepFull) fullNN[n] = 0;
+        }
+    }
+    unsigned long long todo = __ballot(in && cn >= 0 && bufs.slotOf[n] == 0);
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        todo &= todo - 1ull;
+        const int c = __shfl(cn, l, 64);
+        const int nc = min(cnt[c], cap);
+        // the cell's own atoms: lane t holds atom t (ascending) and its position
+        int myI = -1;
+        float4 myA = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < nc) {
+            myI = slots[(size_t)c * cap + lane];
+            myA = cpos[(size_t)c * cap + lane];
+        }
+        const int ccz = c / (p.cx * p.cy);
+        const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
+        const int ccx = c - (ccz * p.cy + ccy) * p.cx;
+        // lane o < 27: neighbour cell o, its count and the exclusive prefix of counts
+        int co = 0;
+        if (lane < 27) {
+            float shx, shy, shz;
+            const int Co = stencil_cell<float>(p, ccx, ccy, ccz, lane, shx, shy, shz);
+            co = min(cnt[Co], cap);
+        }
+        int off = co;  // inclusive scan over the 27 lanes
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const int v = __shfl_up(off, d, 64);
+            if (lane >= d) off += v;
+        }
+        const int total = __shfl(off, 26, 64);
+        off -= co;  // exclusive
+        // per-atom counters live in lane t
+        int nn = 0, nljv = 0, nclv = 0, err = 0;
+        for (int base = 0; base < total; base += 64) {
+            const int j = base + lane;
+            const bool valid = j < total;
+            // the neighbour cell of candidate j: last o with off_o <= j (uniform loop over 27)
+            int o = 0;
+#pragma unroll
+            for (int q = 1; q < 27; ++q)
+                if (__shfl(off, q, 64) <= j) o = q;
+            float shx = 0.f, shy = 0.f, shz = 0.f;
+            const int Co = stencil_cell<float>(p, ccx, ccy, ccz, o, shx, shy, shz);
+            const int slot = j - __shfl(off, o, 64);
+            int k = -1;
+            float4 B = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) {
+                k = slots[(size_t)Co * cap + slot];
+                B = cpos[(size_t)Co * cap + slot];
+            }
+            const float3 Bf = make_float3(B.x, B.y, B.z);
+            for (int t = 0; t < nc; ++t) {
+                const int it = __shfl(myI, t, 64);
+                const float3 At = make_float3(__shfl(myA.x, t, 64), __shfl(myA.y, t, 64), __shfl(myA.z, t, 64));
+                const float rr = shifted_rr(At, Bf, shx, shy, shz);
+                const bool hit = valid && rr < p.rrNebr && k != it;
+                const unsigned long long m = __ballot(hit);
+                if (!m) continue;
+                const int nnT = __shfl(nn, t, 64);
+                const int posn = nnT + __popcll(m & below);
+                if (it < p.nMolMe) {
+                    if (hit) {
+                        if (posn < p.nlCap) {
+                            nlM[(size_t)it * p.nlCap + posn] = k;
+                            if (p.keepFull) fullNBL[(size_t)posn * p.nMol + it] = k;
+                        } else {
+                            err |= SGK_FLAG_NL_OVERFLOW;
+                        }
+                    }
+                } else {
+                    const int ict = it - p.nMolMe;
+                    const unsigned long long mlj = __ballot(hit && k < p.nMolMe);
+                    const unsigned long long mcl = __ballot(hit && rr < p.closeR2);
+                    const int ljT = __shfl(nljv, t, 64), clT = __shfl(nclv, t, 64);
+                    if (hit) {
+                        if (posn < p.nlCap) {
+                            if (p.keepFull) fullNBL[(size_t)posn * p.nMol + it] = k;
+                            const unsigned ent = ((unsigned)k << 8) | (unsigned)posn;
+                            if (k < p.nMolMe) {
+                                const int q = ljT + __popcll(mlj & below);
+                                if (q < p.ljCap)
+                                    lj[(size_t)q * p.nC + ict] = ent;
+                                else
+                                    err |= SGK_FLAG_LJ_OVERFLOW;
+                            }
+                            if (rr < p.closeR2) {
+                                const int q = clT + __popcll(mcl & below);
+                                if (q < p.clCap)
+                                    cl[(size_t)ict * p.clCap + q] = ent;
+                                else
+                                    err |= SGK_FLAG_CLOSE_OVERFLOW;
+                            }
+                        } else {
+                            err |= SGK_FLAG_NL_OVERFLOW;
+                        }
+                    }
+                    if (lane == t) {
+                        nljv += __popcll(mlj);
+                        nclv += __popcll(mcl);
+                    }
+                }
+                if (lane == t) nn += __popcll(m);
+            }
+        }
+        // lane t writes atom t's counts (a cut list past the capacity keeps its first entries,
+        // as the reference's list of iNebrMax entries does, and raises the overflow flag)
+        if (lane < nc) {
+            const int nnc = min(nn, p.nlCap);
+            if (p.keepFull) fullNN[myI] = nnc;
+            if (myI < p.nMolMe) {
+                nnM[myI] = nnc;
+            } else {
+                nlj[myI - p.nMolMe] = min(nljv, p.ljCap);
+                ncl[myI - p.nMolMe] = min(nclv, p.clCap);
+            }
+        }
+        if (err) atomicOr(&st->errors, err);
+    }
+}
+
 template <bool M>
 __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
                                             CellBufs bufs, DevState *st, int force,
@@ -254,7 +408,10 @@ __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>:
                                             unsigned *__restrict__ cl, int *__restrict__ ncl,
                                             int *__restrict__ fullNN, int *__restrict__ fullNBL) {
     const int blk = (p.xcdMap & 1) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    nl_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
+    if (p.nlScheme == 1)
+        nlw_body<M>(p, blk, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
+    else
+        nl_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
 }
 
 // Batched: the step path (lists built from the new cell buffer before the commit).
@@ -263,9 +420,14 @@ __global__ __launch_bounds__(256) void kb_nl(BatchArgs a) {
     const int2 e = a.atomTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
     const DevParams &p = B.p;
-    nl_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
-               a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff,
-               a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
+    if (p.nlScheme == 1)
+        nlw_body<M>(p, e.y, rep_cells(a, B), a.state + e.x, 0, 1, a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff,
+                    a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.cl + B.carbonOff * p.clCap,
+                    a.ncl + B.carbonOff, nullptr, nullptr);
+    else
+        nl_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
+                   a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap,
+                   a.nlj + B.carbonOff, a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
 }
 
 // Make the freshly built buffer current (API path; the step path does this in its reduction).

@@ -36,6 +36,7 @@ struct DevParams {
     int rebuildEvery;
     float dispBound;           // max dispHi for which the carbon close lists are exact
     int xcdMap, binGroup;      // launch-shape options (sgk_config.xcd_map, bin_group)
+    int nlScheme;              // sgk_config.nl_scheme
 };
 
 // Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).

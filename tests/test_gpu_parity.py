@@ -44,12 +44,14 @@ def force_error(acc, ref):
 
 @pytest.mark.parametrize("name", ["small", "icosa", "medium"])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-def test_cells_and_neighbour_list_bit_exact(gpu, name, prec):
+@pytest.mark.parametrize("scheme", [1, 0])
+def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
+    """Both list schemes (one wave per cell, one thread per atom) give the reference list."""
     s = system(name)
     p = s.params
     o = oracle.Oracle(p, "f32")
-    print(f"[parity] cells/NL {name} {prec}", flush=True)
-    with sg.Engine(p, precision=prec, keep_full_nl=True) as eng:
+    print(f"[parity] cells/NL {name} {prec} scheme {scheme}", flush=True)
+    with sg.Engine(p, precision=prec, keep_full_nl=True, nl_scheme=scheme) as eng:
         eng.upload(s.r, s.v, s.a)
         eng.bin()
         cell_of, cnt, slots = eng.get_cells()
