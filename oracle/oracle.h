@@ -11,6 +11,9 @@
  *   *_f64 : double - the FP64 oracle of the same algorithm (same float parameters, promoted)
  * The integer parts (cell index, neighbour list, RDF bins) of *_f32 are bit-exact definitions.
  *
+ * tangent.cpp compiles the same source a third time with forward-mode dual numbers (dual.hpp): the
+ * FP64 values plus directional derivatives, the reference gradient of the adjoint.
+ *
  * Pinning (see tests/test_oracle_*.py): EAM by fcc shell sums (closed form), the zero-force ideal
  * graphene sheet, two-atom LJ, central differences of the conservative EAM+LJ energy, brute-force
  * O(N^2) neighbour lists and RDF, the Berendsen factor, leapfrog on a harmonic dimer, rigid
@@ -50,6 +53,9 @@ typedef struct orc_params {
     /* deposition (regime 1) */
     float velMagDepos;
     int stepDeposit, nMolToDeposit;
+    /* scale factors of the differentiated parameters (1 = the SimParams values): LJ epsilon, LJ
+     * sigma, embedding function F, electron density f */
+    double parScale[4];
 } orc_params;
 
 /* Reductions of one state with the reference semantics (.cu:1101-1891, 2307-2395). */
@@ -131,6 +137,32 @@ int orc_run_f32(const orc_params *p, float *pos4, float *vel, float *acc, int ns
 int orc_run_f64(const orc_params *p, double *pos4, double *vel, double *acc, int nsteps,
                 int stepCount0, int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records,
                 int *nMolDeposited);
+
+/* Ni-C adhesion energy: metal-side LJ pair energies within rrCutLJ, each metal-carbon pair once. */
+float orc_adhesion_f32(const orc_params *p, const float *pos4, const int *NN, const int *NBL);
+double orc_adhesion_f64(const orc_params *p, const double *pos4, const int *NN, const int *NBL);
+
+/* Trajectory window of the adjoint: a(r0) from a list built at r0, then nsteps steps opening with a
+ * rebuild.  J[0] = mean adhesion energy of the steps, J[1] = final metal centre-of-mass height
+ * above z_0.  pos4/vel/acc are updated in place.  Returns 0 or an orc_run error code. */
+int orc_window_f32(const orc_params *p, float *pos4, float *vel, float *acc, int nsteps, int stepCount0,
+                   int rebuildEvery, float *J);
+int orc_window_f64(const orc_params *p, double *pos4, double *vel, double *acc, int nsteps, int stepCount0,
+                   int rebuildEvery, double *J);
+
+/* Tangent (forward-mode) build of the FP64 oracle, tangent.cpp: ORC_ND directions per call.
+ * A direction seeds any of: the four parameter scale factors (pdir[ORC_ND][4]), the initial
+ * positions (rdir[ORC_ND][3*nMol], x y z per atom) and initial velocities (vdir[ORC_ND][3*nMol]);
+ * NULL seeds nothing.  Values are bitwise those of the *_f64 functions. */
+#define ORC_ND 4
+/* orc_window_f64 with derivatives: J[2], dJ[2][ORC_ND]. */
+int orc_tangent_window(const orc_params *p, double *pos4, double *vel, double *acc, int nsteps, int stepCount0,
+                       int rebuildEvery, const double *pdir, const double *rdir, const double *vdir, double *J,
+                       double *dJ);
+/* Density and forces at pos4 with the given list: acc[3*nMol], dacc[ORC_ND][3*nMol], adhesion
+ * energy *adh and dadh[ORC_ND]. */
+void orc_tangent_forces(const orc_params *p, const double *pos4, const int *NN, const int *NBL, const double *pdir,
+                        const double *rdir, double *acc, double *dacc, double *adh, double *dadh);
 
 #ifdef __cplusplus
 }

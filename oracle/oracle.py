@@ -26,8 +26,12 @@ class OrcParams(C.Structure):
         ("deltaT", _f), ("temperature", _f), ("kB", _f), ("gammaBerendsen", _f), ("rNebrShell", _f),
         ("stepThermostat", _i), ("stepEquil", _i), ("stepCool", _i),
         ("rangeRdf", _f), ("intervalRdf", _f), ("sizeHistRdf", _i), ("deltaF", _f),
-        ("velMagDepos", _f), ("stepDeposit", _i), ("nMolToDeposit", _i),
+        ("velMagDepos", _f), ("stepDeposit", _i), ("nMolToDeposit", _i), ("parScale", C.c_double * 4),
     ]
+
+
+ND = 4  # directions per tangent call (ORC_ND)
+PARAMS = ("eps", "sigma", "embed", "density")  # scale factors of the differentiated parameters
 
 
 class OrcProps(C.Structure):
@@ -74,6 +78,8 @@ def lib():
                 "orc_rdf": (C.c_int, [vp, vp, vp, vp, vp, vp]),
                 "orc_diffusion_sample": (C.c_int, [vp, vp, vp]),
                 "orc_run": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
+                "orc_adhesion": (R, [vp, vp, vp, vp]),
+                "orc_window": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
             }
             for name, (res, args) in spec.items():
                 fn = getattr(_lib, f"{name}_{sfx}")
@@ -81,6 +87,10 @@ def lib():
         _lib.orc_diffusion_create.restype = vp
         _lib.orc_diffusion_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp]
         _lib.orc_diffusion_destroy.argtypes = [vp]
+        _lib.orc_tangent_window.restype = C.c_int
+        _lib.orc_tangent_window.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        _lib.orc_tangent_forces.restype = None
+        _lib.orc_tangent_forces.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     return _lib
 
 
@@ -105,6 +115,7 @@ def params_from(sp, cell_cap=64):
     o.lambda_ = sp.lambda_
     o.Fn[:] = list(sp.Fn)
     o.F[:] = list(sp.F)
+    o.parScale[:] = [1.0, 1.0, 1.0, 1.0]
     return o
 
 
@@ -212,3 +223,48 @@ class Oracle:
         out = [{k: (list(getattr(x, k)) if hasattr(getattr(x, k), "__len__") else getattr(x, k))
                 for k, _ in OrcRecord._fields_} for x in recs]
         return err, r, v, a, out
+
+    # ---- adjoint observables and the reference gradient ------------------------------------------
+    def set_scale(self, scale):
+        """Scale factors of (LJ epsilon, LJ sigma, embedding F, density f); (1,1,1,1) = SimParams."""
+        self.p.parScale[:] = [float(x) for x in scale]
+
+    def adhesion(self, r, nn, nbl):
+        return float(self.f("orc_adhesion")(C.byref(self.p), _p(self.arr(r)), _p(nn), _p(nbl)))
+
+    def window(self, r, v, nsteps, step0=0, rebuild_every=0):
+        """Adjoint window (orc_window): returns (J[2], r, v, a, err); J = (mean adhesion energy,
+        final metal centre-of-mass height above z_0)."""
+        r, v = self.arr(r).copy(), self.arr(v).copy()
+        a = np.zeros((self.n, 3), self.dt)
+        J = np.zeros(2, self.dt)
+        err = self.f("orc_window")(C.byref(self.p), _p(r), _p(v), _p(a), nsteps, step0, rebuild_every, _p(J))
+        return J.astype(np.float64), r, v, a, err
+
+    def tangent_window(self, r, v, nsteps, step0=0, rebuild_every=0, pdir=None, rdir=None, vdir=None):
+        """FP64 window with forward-mode derivatives along up to ND directions: pdir (ND,4) seeds the
+        parameter scale factors, rdir/vdir (ND,N,3) the initial positions/velocities.  Returns
+        (J[2], dJ[2,ND], r, v, a, err)."""
+        r = np.ascontiguousarray(r, np.float64).copy()
+        v = np.ascontiguousarray(v, np.float64).copy()
+        a = np.zeros((self.n, 3))
+        J, dJ = np.zeros(2), np.zeros((2, ND))
+        d = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (pdir, rdir, vdir)]
+        for x, shape in zip(d, ((ND, 4), (ND, self.n, 3), (ND, self.n, 3))):
+            if x is not None:
+                assert x.shape == shape, (x.shape, shape)
+        err = self.l.orc_tangent_window(C.byref(self.p), _p(r), _p(v), _p(a), nsteps, step0, rebuild_every,
+                                        _p(d[0]), _p(d[1]), _p(d[2]), _p(J), _p(dJ))
+        return J, dJ, r, v, a, err
+
+    def tangent_forces(self, r, nn, nbl, pdir=None, rdir=None):
+        """Forces with forward-mode derivatives: (acc (N,3), dacc (ND,N,3), adhesion, dadh (ND,))."""
+        r = np.ascontiguousarray(r, np.float64)
+        acc = np.zeros((self.n, 3))
+        dacc = np.zeros((ND, self.n, 3))
+        adh, dadh = C.c_double(), np.zeros(ND)
+        d = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (pdir, rdir)]
+        self.l.orc_tangent_forces(C.byref(self.p), _p(r), _p(np.ascontiguousarray(nn, np.int32)),
+                                  _p(np.ascontiguousarray(nbl, np.int32)), _p(d[0]), _p(d[1]), _p(acc),
+                                  _p(dacc), C.byref(adh), _p(dadh))
+        return acc, dacc, adh.value, dadh

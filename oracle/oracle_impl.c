@@ -15,7 +15,16 @@
 #include <stdlib.h>
 #include <string.h>
 
-#if defined(ORC_F32)
+#if defined(ORC_DUAL) /* tangent build (tangent.cpp): double values, ND derivative directions */
+typedef orc::Dual REAL;
+#define FN(name) name##_dual
+#define EXP orc::exp
+#define POW orc::pow
+#define LOG orc::log
+#define SQRT orc::sqrt
+#define ACOS orc::acos
+#define CEIL orc::ceil
+#elif defined(ORC_F32)
 typedef float REAL;
 #define FN(name) name##_f32
 #define EXP expf
@@ -34,11 +43,27 @@ typedef double REAL;
 #define ACOS acos
 #define CEIL ceil
 #else
-#error "compile with -DORC_F32 or -DORC_F64"
+#error "compile with -DORC_F32, -DORC_F64 or (tangent.cpp) ORC_DUAL"
 #endif
 
 /* Float literals of the reference, promoted so the FP64 oracle evaluates the same model. */
 #define K(x) ((REAL)(x##f))
+
+/* Parameters the adjoint differentiates, each times its scale factor parScale[k] (1 by default,
+ * which leaves every value bitwise unchanged): LJ epsilon and sigma, the embedding function F (all
+ * three branches) and the electron density f.  The tangent build seeds the scale factors. */
+#if defined(ORC_DUAL)
+static REAL g_scale[4];
+#define PSCALE(p, k) (g_scale[k])
+#else
+#define PSCALE(p, k) ((REAL)(p)->parScale[k])
+#endif
+#define PAR_EPS(p) ((REAL)(p)->epsLJ * PSCALE(p, 0))
+#define PAR_SIG(p) ((REAL)(p)->sigmaLJ * PSCALE(p, 1))
+#define PAR_FN(p, k) ((REAL)(p)->Fn[k] * PSCALE(p, 2))
+#define PAR_F(p, k) ((REAL)(p)->F[k] * PSCALE(p, 2))
+#define PAR_FE(p) ((REAL)(p)->Fe * PSCALE(p, 2))
+#define PAR_DENS(p) ((REAL)(p)->fe * PSCALE(p, 3))
 
 /* 27-cell offset table (Defs.h:221-234). */
 static const int kOffsets[27][3] = {
@@ -150,7 +175,7 @@ static inline REAL eam_phi(const orc_params *p, REAL r) {
 }
 static inline REAL eam_f(const orc_params *p, REAL r) {
     REAL x = r * (REAL)p->rei;
-    return (REAL)p->fe * EXP(-(REAL)p->beta * (x - K(1.))) / (K(1.) + POW(x - (REAL)p->lambda, K(20.)));
+    return PAR_DENS(p) * EXP(-(REAL)p->beta * (x - K(1.))) / (K(1.) + POW(x - (REAL)p->lambda, K(20.)));
 }
 static inline REAL eam_dphi(const orc_params *p, REAL r) {
     const REAL rei = p->rei, al = p->alpha, be = p->beta, ka = p->kappa, la = p->lambda;
@@ -165,38 +190,38 @@ static inline REAL eam_df(const orc_params *p, REAL r) {
     const REAL rei = p->rei, be = p->beta, la = p->lambda;
     REAL x = r * rei;
     REAL dB = K(1.) + POW(x - la, K(20.));
-    return -(REAL)p->fe * rei * EXP(-be * (x - K(1.))) * (be + K(20.) * POW(x - la, K(19.)) / dB) / dB;
+    return -PAR_DENS(p) * rei * EXP(-be * (x - K(1.))) * (be + K(20.) * POW(x - la, K(19.)) / dB) / dB;
 }
 static inline REAL eam_F(const orc_params *p, REAL rho) {
     const REAL rhoe = p->rhoe, rhon = K(0.85) * rhoe, rhoo = K(1.15) * rhoe;
     if (rho < rhon) { /* .cu:72-82 */
         REAL rhoni = (REAL)p->rhoei * K(1.176471);
         REAL t = rho * rhoni - K(1.);
-        return (REAL)p->Fn[0] + (REAL)p->Fn[1] * t + (REAL)p->Fn[2] * t * t + (REAL)p->Fn[3] * t * t * t;
+        return PAR_FN(p, 0) + PAR_FN(p, 1) * t + PAR_FN(p, 2) * t * t + PAR_FN(p, 3) * t * t * t;
     }
     if (rho < rhoo) { /* .cu:84-94 */
         REAL t = rho * (REAL)p->rhoei - K(1.);
-        return (REAL)p->F[0] + (REAL)p->F[1] * t + (REAL)p->F[2] * t * t + (REAL)p->F[3] * t * t * t;
+        return PAR_F(p, 0) + PAR_F(p, 1) * t + PAR_F(p, 2) * t * t + PAR_F(p, 3) * t * t * t;
     }
     /* .cu:96-103 */
     REAL x = rho * (REAL)p->rhoei;
-    return (REAL)p->Fe * (K(1.) - LOG(POW(x, (REAL)p->eta))) * POW(x, (REAL)p->eta);
+    return PAR_FE(p) * (K(1.) - LOG(POW(x, (REAL)p->eta))) * POW(x, (REAL)p->eta);
 }
 static inline REAL eam_dF(const orc_params *p, REAL rho) {
     const REAL rhoe = p->rhoe, rhon = K(0.85) * rhoe, rhoo = K(1.15) * rhoe;
     if (rho < rhon) { /* .cu:160-171 */
         REAL rhoni = (REAL)p->rhoei * K(1.176471);
         REAL t = rho * rhoni - K(1.);
-        return rhoni * ((REAL)p->Fn[1] + K(2.) * (REAL)p->Fn[2] * t + K(3.) * (REAL)p->Fn[3] * t * t);
+        return rhoni * (PAR_FN(p, 1) + K(2.) * PAR_FN(p, 2) * t + K(3.) * PAR_FN(p, 3) * t * t);
     }
     if (rho < rhoo) { /* .cu:173-184 */
         REAL x = rho * (REAL)p->rhoei;
-        return (REAL)p->rhoei * ((REAL)p->F[1] + K(2.) * (REAL)p->F[2] * (x - K(1.)) +
-                                 K(3.) * (REAL)p->F[3] * (x - K(1.)) * (x - K(1.)));
+        return (REAL)p->rhoei * (PAR_F(p, 1) + K(2.) * PAR_F(p, 2) * (x - K(1.)) +
+                                 K(3.) * PAR_F(p, 3) * (x - K(1.)) * (x - K(1.)));
     }
     /* .cu:186-195 */
     REAL x = rho * (REAL)p->rhoei, eta = p->eta;
-    return -(REAL)p->Fe * eta * (REAL)p->rhoei * POW(x, (eta - K(1.))) * LOG(POW(x, eta));
+    return -PAR_FE(p) * eta * (REAL)p->rhoei * POW(x, (eta - K(1.))) * LOG(POW(x, eta));
 }
 
 REAL FN(orc_embed)(const orc_params *p, REAL rho, REAL *dF) {
@@ -230,11 +255,11 @@ void FN(orc_rho)(const orc_params *p, const REAL *pos4, const int *NN, const int
 
 /* Lennard-Jones pair (.cu:880-884): returns the force factor, energy in *u. */
 static inline REAL lj_pair(const orc_params *p, REAL rr, REAL *u) {
-    const REAL s2 = (REAL)p->sigmaLJ * (REAL)p->sigmaLJ;
+    const REAL sig = PAR_SIG(p), s2 = sig * sig;
     REAL rri = s2 / rr;
     REAL rri3 = rri * rri * rri;
-    *u = K(4.) * (REAL)p->epsLJ * rri3 * (rri3 - K(1.));
-    return K(48.) * (REAL)p->epsLJ * rri3 * (rri3 - K(0.5)) * rri / s2;
+    *u = K(4.) * PAR_EPS(p) * rri3 * (rri3 - K(1.));
+    return K(48.) * PAR_EPS(p) * rri3 * (rri3 - K(0.5)) * rri / s2;
 }
 
 /* ComputeForcesK (.cu:776-1091), one atom at a time, neighbours in list order. */
@@ -414,9 +439,9 @@ void FN(orc_leapfrog)(const orc_params *p, int part, REAL *pos4, REAL *vel, cons
         }
 }
 
-/* ApplyBerendsenThermostat (.cu:455-485). */
-void FN(orc_thermostat)(const orc_params *p, REAL *vel, double vvSum, int stepCount) {
-    const REAL ke = (REAL)vvSum * K(0.5) / (REAL)p->nMol;
+/* ApplyBerendsenThermostat (.cu:455-485) with the kinetic sum vv of the step. */
+static void FN(thermostat_impl)(const orc_params *p, REAL *vel, REAL vv, int stepCount) {
+    const REAL ke = vv * K(0.5) / (REAL)p->nMol;
     const REAL bC = SQRT(K(1.) + (REAL)p->gammaBerendsen *
                                      ((REAL)p->temperature * K(1.5) * (REAL)p->kB / ke - K(1.)));
     const REAL bM = SQRT(K(1.) + (REAL)p->gammaBerendsen * ((REAL)p->temperature * (REAL)p->kB / ke - K(1.)));
@@ -433,6 +458,34 @@ void FN(orc_thermostat)(const orc_params *p, REAL *vel, double vvSum, int stepCo
         vel[3 * n + 1] = b * vel[3 * n + 1];
         vel[3 * n + 2] = b * vel[3 * n + 2];
     }
+}
+
+void FN(orc_thermostat)(const orc_params *p, REAL *vel, double vvSum, int stepCount) {
+    FN(thermostat_impl)(p, vel, (REAL)vvSum, stepCount);
+}
+
+/* The thermostat's kinetic sum: sum m|v|^2, with the metal centre-of-mass x velocity removed from
+ * the metal vx in regime 2 (ComputeVvSumK, .cu:1230-1234; cmVel.val of .cu:2308). */
+static REAL FN(vv_sum)(const orc_params *p, const REAL *vel) {
+    const int nMol = p->nMol, nMe = p->nMolMe;
+    const REAL m = p->massMe;
+    REAL cmv0 = 0;
+    for (int n = 0; n < nMe; ++n) cmv0 += vel[3 * n];
+#if !defined(ORC_F32)
+    REAL cmVelXr = nMe ? cmv0 / (REAL)nMe : 0;
+#else
+    REAL cmVelXr = nMe ? (float)cmv0 / (float)nMe : 0.f; /* cmVel.val, .cu:2308 */
+#endif
+    REAL vv = 0;
+    for (int n = 0; n < nMol; ++n) {
+        const REAL *v = vel + 3 * n;
+        REAL vx = v[0];
+        if (p->iRegime == 2 && n < nMe) vx = vx - cmVelXr;
+        REAL s = vx * vx + v[1] * v[1] + v[2] * v[2];
+        if (n < nMe) s = m * s;
+        vv += s;
+    }
+    return vv;
 }
 
 /* Reductions (.cu:1101-1891) and the host arithmetic that consumes them (.cu:2307-2395).
@@ -468,31 +521,18 @@ void FN(orc_props)(const orc_params *p, const REAL *pos4, const REAL *vel, const
         if (fric && p->iRegime != 0) fr += fric[n];
     }
     /* regime-2 kinetic energy removes the metal centre-of-mass x velocity (.cu:1230-1234) */
-#if defined(ORC_F64)
-    REAL cmVelXr = nMe ? cmv[0] / (REAL)nMe : 0;
-#else
-    REAL cmVelXr = nMe ? (float)cmv[0] / (float)nMe : 0.f; /* cmVel.val, .cu:2308 */
-#endif
-    REAL vv = 0;
-    for (int n = 0; n < nMol; ++n) {
-        const REAL *v = vel + 3 * n;
-        REAL vx = v[0];
-        if (p->iRegime == 2 && n < nMe) vx = vx - cmVelXr;
-        REAL s = vx * vx + v[1] * v[1] + v[2] * v[2];
-        if (n < nMe) s = m * s;
-        vv += s;
-    }
+    const REAL vv = FN(vv_sum)(p, vel);
     for (int d = 0; d < 3; ++d) {
-        out->vSum[d] = vs[d];
-        out->comSum[d] = com[d];
-        out->cmVelSum[d] = cmv[d];
-        out->bboxMin[d] = nMe ? mn[d] : 0;
-        out->bboxMax[d] = nMe ? mx[d] : 0;
+        out->vSum[d] = (double)vs[d];
+        out->comSum[d] = (double)com[d];
+        out->cmVelSum[d] = (double)cmv[d];
+        out->bboxMin[d] = nMe ? (double)mn[d] : 0;
+        out->bboxMax[d] = nMe ? (double)mx[d] : 0;
     }
-    out->vvSum = vv;
-    out->vvMax = vvMax;
-    out->uSum = u;
-    out->frict = fr;
+    out->vvSum = (double)vv;
+    out->vvMax = (double)vvMax;
+    out->uSum = (double)u;
+    out->frict = (double)fr;
     out->kinEnergy = (double)(K(0.5) * vv / (REAL)nMol);
     out->potEnergy = (double)(u / (REAL)nMol);
     out->totEnergy = (double)((REAL)out->kinEnergy + (REAL)out->potEnergy);
@@ -589,6 +629,7 @@ void orc_diffusion_destroy(orc_diffusion *d) {
 }
 #endif
 
+#if !defined(ORC_DUAL) /* the diffusion buffers hold doubles; not part of the tangent build */
 static inline REAL nint_r(REAL x) { /* Nint (Defs.h:64-65); its 0.5 literal makes this double math */
     double xd = (double)x;
     return (REAL)(xd < 0. ? -(int)(0.5 - xd) : (int)(0.5 + xd));
@@ -632,6 +673,7 @@ int FN(orc_diffusion_sample)(orc_diffusion *d, const REAL *com, REAL *out) {
     }
     return done;
 }
+#endif
 
 /* ------------------------------------------------------------------------------------------ */
 /* InsertAtomsK (.cu:488-514): the group [dep, dep + toDeposit) enters at z = 0.49 Lz moving down
@@ -651,9 +693,32 @@ static void FN(orc_insert)(const orc_params *p, REAL *pos4, REAL *vel, int dep, 
     }
 }
 
-/* The step loop of DoComputationsW (.cu:2141-2489) with device work restated serially. */
-int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nsteps, int stepCount0,
-                int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records, int *nMolDeposited) {
+/* Ni-C adhesion energy: the metal-side Lennard-Jones pair energies of ComputeForcesK (.cu:878-896),
+ * each metal-carbon pair within rrCutLJ once (an observable of the adjoint; no reference output). */
+REAL FN(orc_adhesion)(const orc_params *p, const REAL *pos4, const int *NN, const int *NBL) {
+    const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
+    REAL sum = 0;
+    for (int i = 0; i < p->nMolMe; ++i) {
+        for (int j = 0; j < NN[i]; ++j) {
+            const int k = NBL[(size_t)j * p->nMol + i];
+            if (k < p->nMolMe) continue;
+            REAL dx = wrap1(pos4[4 * i] - pos4[4 * k], Lx);
+            REAL dy = wrap1(pos4[4 * i + 1] - pos4[4 * k + 1], Ly);
+            REAL dz = wrap1(pos4[4 * i + 2] - pos4[4 * k + 2], Lz);
+            REAL rr = dx * dx + dy * dy + dz * dz;
+            REAL u;
+            (void)lj_pair(p, rr, &u);
+            if (rr < (REAL)p->rrCutLJ) sum += u;
+        }
+    }
+    return sum;
+}
+
+/* The step loop of DoComputationsW (.cu:2141-2489) with device work restated serially; adh[s]
+ * (optional) receives the adhesion energy of step s. */
+static int FN(run_impl)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nsteps, int stepCount0,
+                        int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records, int *nMolDeposited,
+                        REAL *adh) {
     const int nMol = p->nMol, nMe = p->nMolMe;
     const int ncell = p->cells[0] * p->cells[1] * p->cells[2];
     int *cellOf = (int *)malloc(sizeof(int) * (size_t)nMol);
@@ -690,12 +755,13 @@ int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nstep
         }
         if (nMe) FN(orc_rho)(p, pos4, NN, NBL, rho);
         FN(orc_forces)(p, pos4, NN, NBL, rho, acc, fric);
+        if (adh) adh[s] = FN(orc_adhesion)(p, pos4, NN, NBL);
         FN(orc_leapfrog)(p, 2, pos4, vel, acc);
         orc_props pr;
         FN(orc_props)(p, pos4, vel, (p->iRegime != 0 && nMe) ? fric : NULL, &pr);
         if ((p->iRegime == 2 && step > p->stepEquil && step % p->stepThermostat == 0) ||
             (p->iRegime == 1 && step % p->stepThermostat == 0))
-            FN(orc_thermostat)(p, vel, pr.vvSum, step);
+            FN(thermostat_impl)(p, vel, FN(vv_sum)(p, vel), step);
         dh += SQRT((REAL)pr.vvMax) * (REAL)p->deltaT;
         if (dh > K(0.5) * (REAL)p->rNebrShell) *nebrNow = 1; /* .cu:2390-2391 */
         if (rebuildEvery > 0 && step % rebuildEvery == 0) *nebrNow = 1; /* fixed cadence adds rebuilds */
@@ -724,12 +790,12 @@ int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nstep
             rec.vvMax = pr.vvMax;
             rec.uSum = pr.uSum;
             rec.frict = pr.frict;
-            rec.dispHi = dh;
+            rec.dispHi = (double)dh;
             rec.nMolDeposited = nMolDeposited ? *nMolDeposited : nMe;
             records[s] = rec;
         }
     }
-    *dispHi = dh;
+    *dispHi = (double)dh;
     free(cellOf);
     free(count);
     free(slots);
@@ -737,5 +803,51 @@ int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nstep
     free(NBL);
     free(rho);
     free(fric);
+    return err;
+}
+
+int FN(orc_run)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nsteps, int stepCount0,
+                int *nebrNow, double *dispHi, int rebuildEvery, orc_record *records, int *nMolDeposited) {
+    return FN(run_impl)(p, pos4, vel, acc, nsteps, stepCount0, nebrNow, dispHi, rebuildEvery, records,
+                        nMolDeposited, NULL);
+}
+
+/* A trajectory window for the adjoint: the acceleration of the initial state from a neighbour list
+ * built at that state, then nsteps steps opening with a rebuild.  J[0] = mean adhesion energy over
+ * the steps, J[1] = metal centre-of-mass height above the sheet plane after the last step. */
+int FN(orc_window)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int nsteps, int stepCount0,
+                   int rebuildEvery, REAL *J) {
+    const int nMol = p->nMol, nMe = p->nMolMe;
+    const int ncell = p->cells[0] * p->cells[1] * p->cells[2];
+    int *cellOf = (int *)malloc(sizeof(int) * (size_t)nMol);
+    int *count = (int *)malloc(sizeof(int) * (size_t)ncell);
+    int *slots = (int *)malloc(sizeof(int) * (size_t)ncell * p->cellCap);
+    int *NN = (int *)calloc((size_t)nMol, sizeof(int));
+    int *NBL = (int *)malloc(sizeof(int) * (size_t)nMol * p->iNebrMax);
+    REAL *rho = (REAL *)calloc((size_t)(nMe ? nMe : 1), sizeof(REAL));
+    REAL *fric = (REAL *)calloc((size_t)(nMe ? nMe : 1), sizeof(REAL));
+    REAL *adh = (REAL *)calloc((size_t)(nsteps > 0 ? nsteps : 1), sizeof(REAL));
+    int err = FN(orc_bin)(p, pos4, cellOf, count, slots);
+    err |= FN(orc_nebr)(p, pos4, count, slots, NN, NBL);
+    if (nMe) FN(orc_rho)(p, pos4, NN, NBL, rho);
+    FN(orc_forces)(p, pos4, NN, NBL, rho, acc, fric);
+    int nebrNow = 1;
+    double dispHi = 0;
+    if (!err)
+        err = FN(run_impl)(p, pos4, vel, acc, nsteps, stepCount0, &nebrNow, &dispHi, rebuildEvery, NULL, NULL,
+                           adh);
+    REAL e = 0, z = 0;
+    for (int s = 0; s < nsteps; ++s) e += adh[s];
+    for (int n = 0; n < nMe; ++n) z += pos4[4 * n + 2];
+    J[0] = nsteps > 0 ? e / (REAL)nsteps : (REAL)0;
+    J[1] = nMe ? z / (REAL)nMe - (REAL)p->z_0 : (REAL)0;
+    free(cellOf);
+    free(count);
+    free(slots);
+    free(NN);
+    free(NBL);
+    free(rho);
+    free(fric);
+    free(adh);
     return err;
 }
