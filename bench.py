@@ -278,6 +278,90 @@ def cpu_baseline(sys_, rebuild, seconds):
             "sample": f"{n} steps (after 1 untimed) of the same workload, FP32 C restatement, 1 thread"}
 
 
+def bench_adjoint(equil, window, every, fd_steps):
+    """Differentiation workload: 8,064 C graphene + 561-atom Ni Mackay icosahedron 3 A above it,
+    600 K, equilibrated `equil` steps on the GPU, then a `window`-step window checkpointed every
+    `every` steps.  Gradients of the window-mean adhesion energy and the final cluster height with
+    respect to the scale factors of LJ epsilon, LJ sigma, embedding F and density f from the GPU
+    adjoint (mixed form), against the tangent build of the FP64 oracle on the same initial state
+    and against central differences of the FP64 oracle at each relative step in fd_steps."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import surfacegrowthconsole_amd as sg
+    from surfacegrowthconsole_amd.configs import rebuild_every
+
+    s = sg.config_system("icosa", "config")
+    p = s.params
+    rebuild = rebuild_every("config")
+    eng = sg.Engine(p, precision="mixed", rebuild_every=rebuild)
+    eng.upload(s.r.astype(np.float64), s.v.astype(np.float64))
+    log(f"[adjoint] equilibrating {equil} steps")
+    left = equil
+    while left > 0:
+        n = min(500, left)
+        eng.run(n, records=False)
+        left -= n
+    r0, v0, _ = eng.download()
+    runs = []
+    for rep in range(2):  # the first pass captures the forward graphs; the second is timed
+        eng.upload(r0, v0)
+        eng.set_run_state(equil, 1, 0.0)
+        t0 = time.perf_counter()
+        obs = eng.adjoint_forward(window, every)
+        t1 = time.perf_counter()
+        g_adh = eng.adjoint_backward([1.0, 0.0])
+        t2 = time.perf_counter()
+        g_h = eng.adjoint_backward([0.0, 1.0])
+        t3 = time.perf_counter()
+        runs.append(dict(obs=obs, grad=np.array([g_adh, g_h]), info=eng.adjoint_info(),
+                         wall_ms=[(t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3]))
+        log(f"[adjoint] pass {rep}: forward {runs[-1]['wall_ms'][0]:.1f} ms, backward "
+            f"{runs[-1]['wall_ms'][1]:.1f} + {runs[-1]['wall_ms'][2]:.1f} ms")
+    run = runs[1]
+    grad = run["grad"]
+    assert np.array_equal(grad, runs[0]["grad"]) or np.allclose(grad, runs[0]["grad"], rtol=1e-12)
+    o = oracle.Oracle(p, "f64")
+    log("[adjoint] tangent oracle")
+    t = time.perf_counter()
+    J, dJ, *_ = o.tangent_window(r0, v0, window, step0=equil, rebuild_every=rebuild, pdir=np.eye(4))
+    t_tan = time.perf_counter() - t
+    out = {
+        "workload": f"icosa: {p.nMol - p.nMolMe} C graphene + {p.nMolMe} Ni icosahedron, 600 K, "
+                    f"equilibrated {equil} steps, window {window} steps, checkpoint every {every}",
+        "atoms": p.nMol,
+        "observables": {"gpu": [run["obs"]["adhesion"], run["obs"]["height"]], "oracle_fp64": J.tolist()},
+        "params": list(oracle.PARAMS),
+        "grad_gpu": grad.tolist(),
+        "grad_tangent_fp64": dJ.tolist(),
+        "rel_err_vs_tangent": (np.abs(grad - dJ) / np.abs(dJ)).tolist(),
+        "gpu_ms": {"forward": run["info"]["forward_ms"], "backward_adhesion": run["wall_ms"][1],
+                   "backward_height": run["wall_ms"][2], "forward_wall": run["wall_ms"][0]},
+        "adjoint_bytes": run["info"]["bytes"],
+        "segments": run["info"]["segments"],
+        "list_copies": run["info"]["list_copies"],
+        "recompute_mismatches": run["info"]["mismatches"],
+        "oracle_tangent_s": t_tan,
+        "fd": {},
+    }
+    for h in fd_steps:
+        log(f"[adjoint] central differences, h = {h:g}")
+        fd = np.zeros((2, 4))
+        for k in range(4):
+            Js = []
+            for sgn in (1, -1):
+                sc = np.ones(4)
+                sc[k] = 1 + sgn * h
+                o.set_scale(sc)
+                Js.append(o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)[0])
+            fd[:, k] = (Js[0] - Js[1]) / (2 * h)
+        o.set_scale(np.ones(4))
+        out["fd"][f"{h:g}"] = {"grad": fd.tolist(), "rel_err_gpu": (np.abs(grad - fd) / np.abs(fd)).tolist(),
+                               "rel_err_tangent": (np.abs(dJ - fd) / np.abs(fd)).tolist()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,9 +380,20 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="NREP",
                     help="time an ensemble of NREP uneven replicas, batched and one context each")
     ap.add_argument("--nve-oracle-steps", type=int, default=1000)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="differentiation workload: adjoint gradients against the FP64 tangent and FD")
+    ap.add_argument("--adjoint-equil", type=int, default=2000)
+    ap.add_argument("--adjoint-window", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--adjoint-fd", default="1e-4,1e-6", help="relative FD steps, comma separated ('' for none)")
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
+        return
+    if args.adjoint:
+        build_if_needed()
+        fd = [float(x) for x in args.adjoint_fd.split(",") if x]
+        print(json.dumps({"adjoint": bench_adjoint(args.adjoint_equil, args.adjoint_window, args.adjoint_every, fd)}))
         return
     if args.batch:
         build_if_needed()

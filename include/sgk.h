@@ -190,6 +190,47 @@ typedef struct sgk_tune_entry {
 } sgk_tune_entry;
 int sgk_autotune(sgk_ctx *ctx, int reps, sgk_tune_entry *entries, int max_entries, int *n_entries);
 
+/* ---- adjoint (no reference counterpart: the reference integrates forward only) -----------------
+ * Gradients of two trajectory observables with respect to four force-field parameters, by reverse-
+ * mode differentiation of the discrete step - hand-written FP64 vector-Jacobian kernels of the
+ * force pass, the leapfrog and the Berendsen thermostat - with step checkpointing and recomputation.
+ * Mixed-precision contexts; regimes 0 and 2 (no deposition; no shear motion inside the window).
+ *   window       sgk_adjoint_forward computes a(r0) from a neighbour list built at the current
+ *                positions, then runs nsteps steps of the device loop opening with a rebuild, with a
+ *                checkpoint (state, lists, run state) every checkpoint_every steps.
+ *   observables  obs[SGK_OBS_ADHESION]: mean over the steps of the Ni-C adhesion energy, the
+ *                metal-side LJ pair energies within the LJ cutoff, each metal-carbon pair once;
+ *                obs[SGK_OBS_HEIGHT]: metal centre-of-mass height above z_0 after the last step.
+ *   parameters   scale factors (at 1) of LJ epsilon, LJ sigma, the embedding function F (all three
+ *                branches) and the electron density f.
+ * Cutoff radii, the F branch points and every discrete decision (lists, bonds, rebuild steps) are
+ * held fixed: the gradient is that of the piecewise-smooth step map along the trajectory, in reduced
+ * units.  records (may be NULL) receives the window's nsteps step records. */
+enum { SGK_PAR_EPSILON = 0, SGK_PAR_SIGMA = 1, SGK_PAR_EMBEDDING = 2, SGK_PAR_DENSITY = 3, SGK_NPAR = 4 };
+enum { SGK_OBS_ADHESION = 0, SGK_OBS_HEIGHT = 1, SGK_NOBS = 2 };
+int sgk_adjoint_forward(sgk_ctx *ctx, int nsteps, int checkpoint_every, double *obs, sgk_record *records);
+/* Reverse sweep for weights w[SGK_NOBS]: grad[SGK_NPAR] = d(w . obs)/d scale; dr0, dv0 (may be NULL)
+ * receive d(w . obs)/d r0 and d/d v0, x y z per atom (lists held fixed).  Recomputes each segment
+ * from its checkpoint and leaves the context in the state the forward ended in; may be repeated
+ * with other weights. */
+int sgk_adjoint_backward(sgk_ctx *ctx, const double *w, double *grad, double *dr0, double *dv0);
+typedef struct sgk_adjoint_info {
+    int nsteps, checkpoint_every, segments;
+    int list_copies;      /* neighbour-list copies held for rebuilds inside a segment */
+    int mismatches;       /* atoms whose recomputed state differed bitwise from the stored one */
+    int pad;
+    long long bytes;      /* device memory of the adjoint */
+    double forward_ms, backward_ms;
+    double obs[2];
+} sgk_adjoint_info;
+int sgk_adjoint_get_info(sgk_ctx *ctx, sgk_adjoint_info *info);
+/* Vector-Jacobian product of the force pass at the current positions and list: for the cotangent
+ * abar[3*nMol] of the accelerations and the weight w_adh of the adhesion energy,
+ * rbar[3*nMol] = d/dr and pbar[SGK_NPAR] = d/d scale of (abar . a(r) + w_adh E_adh(r)). */
+int sgk_force_vjp(sgk_ctx *ctx, const double *abar, double w_adh, double *rbar, double *pbar);
+/* Adhesion energy at the current positions and list (reduced units). */
+int sgk_adhesion_energy(sgk_ctx *ctx, double *e);
+
 /* ---- batched replicas (no reference counterpart: the reference runs one system per process) ----
  * Many replicas of uneven size (cluster size, sheet size, temperature) advance in lockstep with one
  * launch per stage for all of them, steps captured into hipGraphs.  Replicas share the potentials,

@@ -129,6 +129,16 @@ class SgkKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", _i), ("ms", _d)]
 
 
+class SgkAdjointInfo(C.Structure):
+    _fields_ = [("nsteps", _i), ("checkpoint_every", _i), ("segments", _i), ("list_copies", _i),
+                ("mismatches", _i), ("pad", _i), ("bytes", C.c_longlong), ("forward_ms", _d),
+                ("backward_ms", _d), ("obs", _d * 2)]
+
+
+SGK_PARAMS = ("epsilon", "sigma", "embedding", "density")  # SGK_PAR_*: scale factors differentiated
+SGK_OBSERVABLES = ("adhesion", "height")                    # SGK_OBS_*
+
+
 SG_PREC_FP32, SG_PREC_MIXED = 0, 1
 SG_GEOM_REFERENCE, SG_GEOM_SPHERE, SG_GEOM_ICOSAHEDRON = 0, 1, 2
 SG_VEL_REFERENCE, SG_VEL_MAXWELL = 0, 1
@@ -153,6 +163,8 @@ EXPORTED_SYMBOLS = [
     "sgk_batch_download_f64", "sgk_batch_set_run_state", "sgk_batch_run", "sgk_batch_last_run_ms",
     "sgk_batch_rdf_get", "sgk_batch_rdf_clear", "sgk_batch_device_bytes", "sgk_batch_metal_lanes",
     "sgk_batch_atoms",
+    "sgk_adjoint_forward", "sgk_adjoint_backward", "sgk_adjoint_get_info", "sgk_force_vjp",
+    "sgk_adhesion_energy",
 ]
 
 _lib = None
@@ -229,6 +241,11 @@ def _proto(lib):
         "sgk_batch_device_bytes": (C.c_longlong, [vp]),
         "sgk_batch_metal_lanes": (_i, [vp]),
         "sgk_batch_atoms": (_i, [vp]),
+        "sgk_adjoint_forward": (_i, [vp, _i, _i, vp, vp]),
+        "sgk_adjoint_backward": (_i, [vp, vp, vp, vp, vp]),
+        "sgk_adjoint_get_info": (_i, [vp, P(SgkAdjointInfo)]),
+        "sgk_force_vjp": (_i, [vp, vp, C.c_double, vp, vp]),
+        "sgk_adhesion_energy": (_i, [vp, dp]),
     }
     for name, (res, args) in spec.items():
         fn = getattr(lib, name)
@@ -259,6 +276,6 @@ def check_layouts(l):
     if list(out[:n]) != mine[:n]:
         raise RuntimeError(f"ABI struct size mismatch: library {list(out[:n])} vs bindings {mine[:n]}")
     n = l.sgk_struct_sizes(out, 8)
-    mine = [C.sizeof(SgkConfig), C.sizeof(SgkRecord), C.sizeof(SgkKernelStat)]
+    mine = [C.sizeof(SgkConfig), C.sizeof(SgkRecord), C.sizeof(SgkKernelStat), C.sizeof(SgkAdjointInfo)]
     if list(out[:n]) != mine[:n]:
         raise RuntimeError(f"sgk struct size mismatch: library {list(out[:n])} vs bindings {mine[:n]}")

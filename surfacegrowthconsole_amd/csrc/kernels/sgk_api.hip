@@ -178,6 +178,7 @@ void sgk::make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &
     p.epsLJ = s.epsLJ;
     p.sigma2 = s.sigmaLJ * s.sigmaLJ;
     p.inv_sigma2 = 1.f / p.sigma2;
+    p.sigmaLJ = s.sigmaLJ;
     p.deltaT = s.deltaT;
     p.temperature = s.temperature;
     p.kB = s.kB;
@@ -367,6 +368,10 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
 int sgk_destroy(sgk_ctx *c) {
     if (!c) return SGK_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->adj) {
+        sgk::adjoint_free(c->adj);
+        delete c->adj;
+    }
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
@@ -737,9 +742,12 @@ int sgk_rdf_get(sgk_ctx *c, unsigned int *hist) {
     return SGK_OK;
 }
 
+}  // extern "C"
+
 // ---- device-resident loop -----------------------------------------------------------------------------
-// Per-step kernel-sequence flags known on the host: thermostat launch, shear launch, RDF launch.
-static std::string step_flags(sgk_ctx *c, int step0, int nsteps, int rdfEvery) {
+// Per-step kernel-sequence flags known on the host: thermostat launch, shear launch, RDF launch,
+// adhesion-energy record (adjoint windows).
+std::string sgk::step_flag_string(sgk_ctx *c, int step0, int nsteps, int rdfEvery, bool observe) {
     const SimParams &s = c->sp;
     std::string f(nsteps, 'a');
     for (int k = 0; k < nsteps; ++k) {
@@ -750,7 +758,7 @@ static std::string step_flags(sgk_ctx *c, int step0, int nsteps, int rdfEvery) {
         // with deltaF = 0 the shear stays 0 and ApplyShearK adds nothing: no launch needed
         const bool shear = s.iRegime == 2 && s.nMolMe > 0 && step > s.stepEquil + s.stepCool && s.deltaF != 0.f;
         const bool rdf = rdfEvery > 0 && step % rdfEvery == 0;
-        b = (thermo ? 1 : 0) | (shear ? 2 : 0) | (rdf ? 4 : 0);
+        b = (thermo ? 1 : 0) | (shear ? 2 : 0) | (rdf ? 4 : 0) | (observe ? 8 : 0);
         f[k] = (char)('a' + b);
     }
     return f;
@@ -769,15 +777,28 @@ static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
         if (b & 1) sgk::launch_thermostat(c, 0, 0.0, 0);
         if (b & 2) sgk::launch_shear(c);
         if (b & 4) sgk::launch_rdf(c);
+        if (b & 8) sgk::launch_adh_obs(c, c->adj->d.part, c->adj->d.adh, 1);
     }
 }
 
+int sgk::set_error(sgk_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+int sgk::check_device_flags(sgk_ctx *c) { return check_flags(c); }
+
+extern "C" {
+
 int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
+    return sgk::run_steps(c, nsteps, rdfEvery, records, nullptr);
+}
+
+}  // extern "C"
+
+int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, double *adh) {
     if (!c) return SGK_E_ARG;
     if (nsteps <= 0) return SGK_OK;
     if (nsteps > c->cfg.records_capacity)
         return fail(c, SGK_E_ARG, "nsteps exceeds records_capacity; call sgk_run in chunks");
-    const std::string flags = step_flags(c, c->hostStep, nsteps, rdfEvery);
+    if (adh && (!c->adj || c->adj->every < nsteps)) return fail(c, SGK_E_STATE, "no adhesion-energy buffer for the run");
+    const std::string flags = sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, adh != nullptr);
     SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (c->cfg.use_graphs && !c->profiling) {
         auto it = c->graphs.find(flags);
@@ -812,6 +833,8 @@ int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
     if (records)
         SGK_HIP(c, hipMemcpyAsync(records, c->records, sizeof(sgk_record) * (size_t)nsteps, hipMemcpyDeviceToHost,
                                   c->stream));
+    if (adh)
+        SGK_HIP(c, hipMemcpyAsync(adh, c->adj->d.adh, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));
     SGK_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
@@ -822,6 +845,8 @@ int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
     c->haveCells = false;  // cells are those of the last rebuild, not of the current positions
     return check_flags(c);
 }
+
+extern "C" {
 
 double sgk_last_run_ms(sgk_ctx *c) { return c ? c->lastRunMs : 0.0; }
 
@@ -851,9 +876,10 @@ int sgk_kernel_stats(sgk_ctx *c, sgk_kernel_stat *out, int max) {
 long long sgk_device_bytes(sgk_ctx *c) { return c ? c->bytes : 0; }
 
 int sgk_struct_sizes(int *out, int max) {
-    const int s[] = {(int)sizeof(sgk_config), (int)sizeof(sgk_record), (int)sizeof(sgk_kernel_stat)};
+    const int s[] = {(int)sizeof(sgk_config), (int)sizeof(sgk_record), (int)sizeof(sgk_kernel_stat),
+                     (int)sizeof(sgk_adjoint_info)};
     int n = 0;
-    for (; n < max && n < 3; ++n) out[n] = s[n];
+    for (; n < max && n < 4; ++n) out[n] = s[n];
     return n;
 }
 
@@ -872,11 +898,15 @@ static void set_metal_lanes(sgk_ctx *c, int G) {
     c->nbRho = c->nbM;
 }
 
-static void drop_graphs(sgk_ctx *c) {
+}  // extern "C"
+
+void sgk::drop_graphs(sgk_ctx *c) {
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     c->graphs.clear();
 }
+
+extern "C" {
 
 int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, int *nEntries) {
     if (!c) return SGK_E_ARG;
@@ -928,7 +958,7 @@ int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, 
     sgk::launch_rho(c);
     sgk::launch_force(c, false, false);
     SGK_HIP(c, hipStreamSynchronize(c->stream));
-    drop_graphs(c);  // captured launches carry the old grid and parameters
+    sgk::drop_graphs(c);  // captured launches carry the old grid and parameters
     c->profiling = prof;
     if (nEntries) *nEntries = n;
     return check_flags(c);

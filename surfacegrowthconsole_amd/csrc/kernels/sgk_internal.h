@@ -3,11 +3,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "sgk.h"
+#include "sgk_adjoint.h"
 #include "sgk_batch.h"
 #include "sgk_params.h"
 
@@ -68,6 +70,7 @@ struct sgk_ctx {
     bool profiling = false;
     std::map<std::string, sgk::KernelStat> stats;
     std::string err;
+    sgk::Adjoint *adj = nullptr;                        // adjoint storage (sgk_adjoint_*), on demand
 };
 
 // A batch of replicas advanced together (sgk_batch_*): shared arrays, one range per replica.
@@ -174,6 +177,16 @@ void launch_finalize(sgk_ctx *c, int mode);          // 0 step, 1 evaluate only
 void launch_thermostat(sgk_ctx *c, int stepOverride, double vvOverride, int useOverride);
 void launch_shear(sgk_ctx *c);
 void launch_reset_records(sgk_ctx *c);
+
+// sgk_api.hip: error reporting, device flags, the step-kernel flags of a run and the device loop.
+int set_error(sgk_ctx *c, int code, const std::string &msg);
+int check_device_flags(sgk_ctx *c);
+// One character per step, 'a' + bits: 1 thermostat, 2 shear, 4 RDF, 8 adhesion energy recorded.
+std::string step_flag_string(sgk_ctx *c, int step0, int nsteps, int rdfEvery, bool observe);
+// sgk_run with, if adh is not null, the adhesion energy of every step copied to adh[nsteps].
+int run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, double *adh);
+// Destroy the cached step graphs (their launches carry grids, parameters and buffer addresses).
+void drop_graphs(sgk_ctx *c);
 
 // Per-kernel timing hook used when profiling is on (sgk_api.hip).
 void profile_begin(sgk_ctx *c);

@@ -37,6 +37,7 @@ struct DevParams {
     float dispBound;           // max dispHi for which the carbon close lists are exact
     int xcdMap, binGroup;      // launch-shape options (sgk_config.xcd_map, bin_group)
     int nlScheme;              // sgk_config.nl_scheme
+    float sigmaLJ;             // LJ sigma (the FP64 derivatives of the adjoint square it in FP64)
 };
 
 // Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).

@@ -262,6 +262,52 @@ class Engine:
     def device_bytes(self):
         return self.lib.sgk_device_bytes(self.h)
 
+    # -- adjoint (mixed precision) -------------------------------------------------------------
+    def adjoint_forward(self, nsteps, checkpoint_every=20, records=False):
+        """Run a differentiation window from the current state (sgk_adjoint_forward): returns
+        {"adhesion": mean Ni-C adhesion energy, "height": final metal centre-of-mass height above
+        the sheet plane} in reduced units, plus the step records if asked."""
+        obs = np.zeros(2)
+        recs = (N.SgkRecord * nsteps)() if records else None
+        self._check(self.lib.sgk_adjoint_forward(self.h, nsteps, checkpoint_every, _ptr(obs), recs),
+                    "sgk_adjoint_forward")
+        out = dict(zip(N.SGK_OBSERVABLES, obs.tolist()))
+        if records:
+            out["records"] = [record_dict(r) for r in recs]
+        return out
+
+    def adjoint_backward(self, weights, initial_state=False):
+        """Gradient of weights . (adhesion, height) with respect to the scale factors of
+        (LJ epsilon, LJ sigma, embedding F, density f); with initial_state also the gradients with
+        respect to the window's initial positions and velocities, (N,3) each."""
+        w = np.ascontiguousarray(weights, np.float64)
+        g = np.zeros(4)
+        dr0 = np.zeros((self.n, 3)) if initial_state else None
+        dv0 = np.zeros((self.n, 3)) if initial_state else None
+        self._check(self.lib.sgk_adjoint_backward(self.h, _ptr(w), _ptr(g), _ptr(dr0), _ptr(dv0)),
+                    "sgk_adjoint_backward")
+        return (g, dr0, dv0) if initial_state else g
+
+    def adjoint_info(self):
+        info = N.SgkAdjointInfo()
+        self._check(self.lib.sgk_adjoint_get_info(self.h, C.byref(info)), "sgk_adjoint_get_info")
+        return {k: (list(getattr(info, k)) if k == "obs" else getattr(info, k)) for k, _ in info._fields_
+                if k != "pad"}
+
+    def force_vjp(self, abar, w_adhesion=0.0):
+        """(d/dr (N,3), d/d scale (4,)) of abar . a(r) + w_adhesion E_adh(r) at the current state."""
+        ab = np.ascontiguousarray(abar, np.float64)
+        rbar = np.zeros((self.n, 3))
+        pbar = np.zeros(4)
+        self._check(self.lib.sgk_force_vjp(self.h, _ptr(ab), float(w_adhesion), _ptr(rbar), _ptr(pbar)),
+                    "sgk_force_vjp")
+        return rbar, pbar
+
+    def adhesion_energy(self):
+        e = C.c_double()
+        self._check(self.lib.sgk_adhesion_energy(self.h, C.byref(e)), "sgk_adhesion_energy")
+        return e.value
+
 
 class Batch:
     """Many replicas advanced together (sgk_batch): one launch per stage for all of them.  Replicas
