@@ -200,7 +200,6 @@ int restore(sgk_ctx *c, const Checkpoint &k) {
         s.cellbuf = c->hostState->cellbuf;
         s.rebuilds = c->hostState->rebuilds;
     }
-    s.blocksDone = 0;
     *c->hostState = s;
     ADJ_HIP(c, hipMemcpyAsync(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
     ADJ_HIP(c, hipStreamSynchronize(c->stream));

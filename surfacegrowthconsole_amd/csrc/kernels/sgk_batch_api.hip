@@ -101,6 +101,7 @@ void enqueue_batch(sgk_batch *b, const std::string &flags) {
         sgk::launch_batch_nl(b);
         sgk::launch_batch_rho(b);
         sgk::launch_batch_force(b);
+        sgk::launch_batch_finalize(b);
         if (k & 1) sgk::launch_batch_thermostat(b);
         if (k & 2) sgk::launch_batch_shear(b);
         if (k & 4) sgk::launch_batch_rdf(b);
@@ -361,7 +362,6 @@ int sgk_batch_set_run_state(sgk_batch *b, int rep, int stepCount, int nebrNow, d
     s.dispHi = dispHi;
     s.errors = 0;
     s.rec = 0;
-    s.blocksDone = 0;
     BHIP(b, hipMemcpy(b->state + rep, &s, sizeof(DevState), hipMemcpyHostToDevice));
     if (rep == 0) b->hostStep = stepCount;
     return SGK_OK;

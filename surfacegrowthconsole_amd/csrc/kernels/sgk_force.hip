@@ -12,8 +12,10 @@
 //   k_force  one launch: blocks [0, nbM) are the metal pass (G lanes per atom: EAM pairs + LJ with
 //            carbon), blocks [nbM, nbM + nbC) the carbon pass (one thread per atom: bond, plane and
 //            angle terms from the close list + LJ with metal).  The epilogue writes acceleration,
-//            energy, friction, applies leapfrog part 2 and leaves one Partial per block; the last
-//            block to finish reduces them and runs the step bookkeeping (finalize_body).
+//            energy, friction, applies leapfrog part 2 and leaves one Partial per block; k_finalize
+//            (sgk_step.hip), the next launch, reduces them and runs the step bookkeeping.  (A
+//            last-block reduction inside k_force needs a device-scope release fence per block, which
+//            on gfx950 writes back and invalidates the XCD's L2 - it cost more than the pass itself.)
 // Precision: displacement vectors, cutoff tests and, for the stiff graphene terms, the bond-length
 // and angle deviations are formed in the storage precision R (FP64 in the mixed form).  EAM and LJ
 // pair arithmetic runs in T (FP32, or FP64 with pair_fp64); graphene force-law arithmetic is FP32.
@@ -288,9 +290,10 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
 // Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form:
 // sin(theta - 2pi/3) and cos(theta - 2pi/3) are formed in FP64 (the sine cancels near the
 // equilibrium angle, so it needs the extra precision), the arc from them in FP32 - the deviation is
-// an FP32 quantity of the force law, and atan2f of two correctly signed values carries a relative
-// error of a few ulp - plus the constant (2pi/3 - theta_0) because theta_0 is the float 2pi/3 of
-// the reference.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
+// an FP32 quantity of the force law - plus the constant (2pi/3 - theta_0) because theta_0 is the
+// float 2pi/3 of the reference.  Within 45 degrees of 2pi/3 (every bonded angle of a sheet that
+// holds together) the arc is atan of u/v by polynomial, elsewhere atan2f; both carry a relative
+// error of a few ulp.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
 template <class R>
 struct AngleDev;
 template <>
@@ -303,12 +306,14 @@ struct AngleDev<float> {
 template <>
 struct AngleDev<double> {
     __device__ static __forceinline__ float eval(double c, const DevParams &p, float &s) {
-        const double sd = sqrt(1.0 - c * c);
+        const double sd = sqrt64(1.0 - c * c);
         const double h = 0.86602540378443864676;  // sin(2pi/3)
         const double u = -0.5 * sd - h * c;        // sin(theta - 2pi/3)
         const double v = -0.5 * c + h * sd;        // cos(theta - 2pi/3)
         s = (float)sd;
-        return (float)((double)atan2f((float)u, (float)v) + (2.0943951023931954923 - (double)p.theta_0));
+        const float fu = (float)u, fv = (float)v;
+        const float a = (fv > 0.f && fabsf(fu) <= fv) ? atan_unit(fu * rcpf(fv)) : atan2f(fu, fv);
+        return (float)((double)a + (2.0943951023931954923 - (double)p.theta_0));
     }
 };
 
@@ -317,9 +322,11 @@ struct V3 {
     R x, y, z;
 };
 
-// Reciprocal in the displacement precision R.
+// Reciprocal and square root in the displacement precision R.
 __device__ __forceinline__ float rcpR(float x) { return rcpf(x); }
 __device__ __forceinline__ double rcpR(double x) { return rcp64(x); }
+__device__ __forceinline__ float sqrtR(float x) { return sqrtf(x); }
+__device__ __forceinline__ double sqrtR(double x) { return sqrt64(x); }
 
 // Neighbour-angle terms of bond (i,k) with k's bonded neighbour m (SurfaceGrowth.cu:953-978).
 // rik: r_i - r_k, aik = |rik|; e: r_m - r_k, rr2 = |e|^2 (the reference divides by rr2 where |e|
@@ -381,7 +388,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     for (int b = 0; b < 3; ++b) {
         if (b >= counti) continue;
         const R rrr = disp<R>(Ai, Bk[b], p, rnn[b].x, rnn[b].y, rnn[b].z);
-        absR[b] = sqrt(rrr);
+        absR[b] = sqrtR(rrr);
         dZ[b] = (float)(Bk[b].z - z0);
     }
     // For each bond k: k's own near neighbours m != i, the first two in k's list order (.cu:940-978).
@@ -488,8 +495,8 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     return q;
 }
 
-// One workgroup of the force launch (blocks [0, nbM) metal, [nbM, nb) carbon).  With st set, the
-// last of the nb workgroups to finish reduces the nb partials and runs the step bookkeeping.
+// One workgroup of the force launch (blocks [0, nbM) metal, [nbM, nb) carbon): forces, energies,
+// leapfrog part 2 and the block's Partial in part[b].
 template <bool M, class T, int G>
 __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, int nb, bool xm,
                                             typename Prec<M>::P4 *__restrict__ pos,
@@ -499,7 +506,7 @@ __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, 
                                             float *__restrict__ fric, const unsigned *__restrict__ lj,
                                             const int *__restrict__ nlj, const uint4 *__restrict__ bondC,
                                             const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
-                                            int integrate, DevState *st, sgk_record *records, int recCap) {
+                                            int integrate) {
     using A = typename Prec<M>::A;
     Partial<A> q;
     if (b < nbM)
@@ -509,20 +516,7 @@ __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, 
         q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, vel, acc, lj, nlj, bondC, bondA,
                               integrate);
     const Partial<A> r = b < nbM ? block_reduce_partial<true>(q) : block_reduce_partial<false>(q);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        part[b] = r;
-        last = 0;
-        if (st) {
-            __threadfence();
-            last = atomicAdd(&st->blocksDone, 1u) == (unsigned)nb - 1u;
-        }
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    finalize_body<A>(p, part, nb, st, records, recCap, 0);
-    if (threadIdx.x == 0) st->blocksDone = 0;
+    if (threadIdx.x == 0) part[b] = r;
 }
 
 template <bool M, class T, int G>
@@ -533,13 +527,13 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
                                                float *__restrict__ fric, const unsigned *__restrict__ lj,
                                                const int *__restrict__ nlj, const uint4 *__restrict__ bondC,
                                                const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
-                                               int integrate, DevState *st, sgk_record *records, int recCap) {
+                                               int integrate) {
     // nbM is a multiple of 8: both ranges keep the XCD of each block
     force_block<M, T, G>(p, blockIdx.x, nbM, gridDim.x, p.xcdMap & 4, pos, vel, acc, nlM, nnM, embF, embDF, fric,
-                         lj, nlj, bondC, bondA, part, integrate, st, records, recCap);
+                         lj, nlj, bondC, bondA, part, integrate);
 }
 
-// Batched: each replica's workgroups reduce into its own partials, state and records.
+// Batched: each replica's workgroups write into its own range of partials (kb_finalize reduces).
 template <bool M, class T, int G>
 __global__ __launch_bounds__(256) void kb_force(BatchArgs a) {
     using R = typename Prec<M>::R;
@@ -551,8 +545,7 @@ __global__ __launch_bounds__(256) void kb_force(BatchArgs a) {
                          (R *)a.vel + 3 * B.atomOff, (R *)a.acc + 3 * B.atomOff, a.nlM + B.metalOff * p.nlCap,
                          a.nnM + B.metalOff, a.embF + B.metalOff, a.embDF + B.metalOff, a.fric + B.metalOff,
                          a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.bondC + B.carbonOff,
-                         a.bondA + B.carbonOff, (Partial<A> *)a.partials + B.partOff, 1, a.state + e.x,
-                         a.records + (size_t)e.x * a.recCap, a.recCap);
+                         a.bondA + B.carbonOff, (Partial<A> *)a.partials + B.partOff, 1);
 }
 
 // ---- launchers ---------------------------------------------------------------------------------
@@ -577,7 +570,7 @@ static void rho_dispatch(sgk_ctx *c) {
 }
 
 template <bool M, class T>
-static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
+static void force_dispatch(sgk_ctx *c, bool integrate) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
@@ -586,11 +579,10 @@ static void force_dispatch(sgk_ctx *c, bool integrate, bool finalize) {
     P4 *pos = (P4 *)c->pos;
     R *vel = (R *)c->vel;
     Partial<A> *part = (Partial<A> *)c->partials;
-    DevState *st = finalize ? c->state : nullptr;
 #define SGK_F(GG)                                                                                                \
     hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, (R *)c->acc, \
                        c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->bondC, c->bondA, part,     \
-                       (int)integrate, st, c->records, c->cfg.records_capacity)
+                       (int)integrate)
     switch (G) {
         case 1: SGK_F(1); break;
         case 2: SGK_F(2); break;
@@ -650,12 +642,13 @@ void launch_force(sgk_ctx *c, bool integrate, bool finalize) {
     if (c->nbM + c->nbC == 0) return;
     profile_begin(c);
     if (!c->mixed)
-        force_dispatch<false, float>(c, integrate, finalize);
+        force_dispatch<false, float>(c, integrate);
     else if (c->cfg.pair_fp64)
-        force_dispatch<true, double>(c, integrate, finalize);
+        force_dispatch<true, double>(c, integrate);
     else
-        force_dispatch<true, float>(c, integrate, finalize);
+        force_dispatch<true, float>(c, integrate);
     profile_end(c, "force");
+    if (finalize) launch_finalize(c, 0);
 }
 
 }  // namespace sgk

@@ -168,6 +168,7 @@ void launch_force(sgk_ctx *c, bool integrate, bool finalize);
 void launch_lf1(sgk_ctx *c, bool bin);
 void launch_insert(sgk_ctx *c);
 void launch_batch_lf1(sgk_batch *b);
+void launch_batch_finalize(sgk_batch *b);
 void launch_batch_thermostat(sgk_batch *b);
 void launch_batch_shear(sgk_batch *b);
 void launch_batch_reset_records(sgk_batch *b);

@@ -73,7 +73,6 @@ struct DevState {
     double cmVelX;         // metal centre-of-mass velocity x of the last step
     unsigned int shearCount;
     int nMolDeposited;     // metal atoms deposited so far (regime 1)
-    unsigned int blocksDone;  // force-kernel blocks finished in this step (last-block reduction)
 };
 
 }  // namespace sgk

@@ -1,5 +1,5 @@
-// sgk_reduce.h - per-block partial reductions and the per-step bookkeeping (device code shared by
-// the force kernel, which runs it in its last finishing block, and the standalone reduction).
+// sgk_reduce.h - per-block partial reductions (the force and props kernels) and the per-step
+// bookkeeping that k_finalize / kb_finalize run over them.
 #pragma once
 
 #include "sgk.h"

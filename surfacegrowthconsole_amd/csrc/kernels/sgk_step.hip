@@ -157,12 +157,24 @@ __global__ __launch_bounds__(256) void k_props(DevParams p, const typename Prec<
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// Standalone reduction of the partials (the step loop runs finalize_body inside k_force).
+// Reduction of the partials of the force pass (mode 0: the step's bookkeeping) or of k_props
+// (mode 1), one workgroup of kFinalizeThreads.
+constexpr int kFinalizeThreads = 1024;
 template <bool M>
-__global__ __launch_bounds__(256) void k_finalize(DevParams p, const Partial<typename Prec<M>::A> *__restrict__ part,
-                                                  int nParts, DevState *st, sgk_record *records, int recCap,
-                                                  int mode) {
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(DevParams p,
+                                                               const Partial<typename Prec<M>::A> *__restrict__ part,
+                                                               int nParts, DevState *st, sgk_record *records,
+                                                               int recCap, int mode) {
     finalize_body<typename Prec<M>::A>(p, part, nParts, st, records, recCap, mode);
+}
+
+// Batched: one workgroup per replica, the same reduction as k_finalize over the replica's partials.
+template <bool M>
+__global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a) {
+    using A = typename Prec<M>::A;
+    const BatchRep &B = a.reps[blockIdx.x];
+    finalize_body<A>(B.p, (const Partial<A> *)a.partials + B.partOff, B.nbForce, a.state + blockIdx.x,
+                     a.records + (size_t)blockIdx.x * a.recCap, a.recCap, 0);
 }
 
 // ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485), kinetic energy of all atoms.
@@ -311,10 +323,10 @@ void launch_finalize(sgk_ctx *c, int mode) {
     const int nParts = mode == 1 ? c->nbProps : c->nbM + c->nbC;
     const int cap = mode == 1 ? 1 : c->cfg.records_capacity;
     if (c->mixed)
-        hipLaunchKernelGGL(k_finalize<true>, dim3(1), dim3(256), 0, c->stream, c->dp,
+        hipLaunchKernelGGL(k_finalize<true>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp,
                            (const Partial<double> *)c->partials, nParts, c->state, c->records, cap, mode);
     else
-        hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(256), 0, c->stream, c->dp,
+        hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp,
                            (const Partial<float> *)c->partials, nParts, c->state, c->records, cap, mode);
     profile_end(c, "finalize");
 }
@@ -353,6 +365,13 @@ void launch_batch_lf1(sgk_batch *b) {
         hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
     else
         hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+}
+
+void launch_batch_finalize(sgk_batch *b) {
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
+    else
+        hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
 }
 
 void launch_batch_thermostat(sgk_batch *b) {
