@@ -338,7 +338,8 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const size_t nbMmax = (size_t)((((long long)p.nMolMe * 16 + 255) / 256 + 7) / 8 * 8);
     const size_t nParts = std::max<size_t>(std::max<size_t>(nbMmax, (size_t)c->nbM) + (size_t)c->nbC,
                                            (size_t)c->nbProps);
-    SGK_TRY(dalloc(c, (char **)&c->partials, nParts * sgk::partial_bytes(c), true));
+    c->part2Off = (int)nParts;  // then one chunk total per 64 partials (two-stage step reduction)
+    SGK_TRY(dalloc(c, (char **)&c->partials, (nParts + nParts / 64 + 2) * sgk::partial_bytes(c), true));
     SGK_TRY(dalloc(c, &c->state, 1, true));
     SGK_TRY(dalloc(c, &c->records, (size_t)cfg.records_capacity, true));
     SGK_TRY(dalloc(c, &c->hist, (size_t)p.sizeHistRdf, true));

@@ -21,6 +21,8 @@ struct BatchRep {
     int partOff;         // first force-pass partial
     int nbForce;         // force-pass workgroups: nbM metal then carbon
     int nbM;             // metal workgroups of the density and force passes
+    int part2Off;        // first chunk total of the two-stage step reduction
+    int nChunks;         // chunks of kPartChunk force-pass partials
     int pad;
 };
 

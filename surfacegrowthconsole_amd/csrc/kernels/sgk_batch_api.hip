@@ -209,6 +209,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         const int nbC = (R.p.nC + 255) / 256;
         R.nbForce = R.nbM + nbC;
         R.partOff = partOff;
+        R.nChunks = (R.nbForce + 63) / 64;  // sgk::kPartChunk partials per chunk
         for (int k = 0; k < nbAtom; ++k) atomTab.push_back(make_int2(r, k));
         for (int k = 0; k < R.nbForce; ++k) forceTab.push_back(make_int2(r, k));
         for (int k = 0; k < nbMetal; ++k) metalTab.push_back(make_int2(r, k));
@@ -218,6 +219,10 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         carbonOff += R.p.nC;
         cellOff += R.p.ncell;
         partOff += R.nbForce;
+    }
+    for (sgk::BatchRep &R : b->reps) {  // chunk totals after all partials
+        R.part2Off = partOff;
+        partOff += R.nChunks;
     }
     b->nAtoms = atomOff;
     b->nMetal = metalOff;

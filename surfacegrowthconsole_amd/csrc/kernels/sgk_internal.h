@@ -60,6 +60,7 @@ struct sgk_ctx {
     sgk::DevState *hostState = nullptr;                 // pinned mirror
     // launch geometry
     int block = 256, metalLanes = 8, nbM = 0, nbC = 0, nbRho = 0, nbProps = 0;
+    int part2Off = 0;                                   // chunk totals of the step reduction (in partials)
     long long bytes = 0;
     bool haveList = false;
     bool haveCells = false;                             // sgk_bin ran on the current positions

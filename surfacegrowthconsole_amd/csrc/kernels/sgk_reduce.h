@@ -71,6 +71,19 @@ __device__ Partial<T> block_reduce_partial(Partial<T> q) {
     return r;
 }
 
+// First stage of the step reduction: one wave reduces a chunk of up to kPartChunk partials (lane k
+// takes partial k; fixed order), the result goes to *out.  The chunk count is wave-uniform.
+constexpr int kPartChunk = 64;
+template <class T>
+__device__ __forceinline__ void reduce_chunk(const Partial<T> *part, int n, Partial<T> *out) {
+    const int lane = threadIdx.x & 63;
+    Partial<T> q;
+    partial_identity(q);
+    if (lane < n) q = part[lane];
+    partial_wave<true>(q);
+    if (lane == 63) *out = q;
+}
+
 // Reduce nParts partials with the whole block (each thread a strided subset, then the block tree:
 // the order is fixed, so the result is bitwise reproducible), then evaluate the reference's host
 // arithmetic of SurfaceGrowth.cu:2307-2395 in thread 0.

@@ -157,9 +157,19 @@ __global__ __launch_bounds__(256) void k_props(DevParams p, const typename Prec<
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// Reduction of the partials of the force pass (mode 0: the step's bookkeeping) or of k_props
-// (mode 1), one workgroup of kFinalizeThreads.
-constexpr int kFinalizeThreads = 1024;
+// The step reduction in two stages: k_reduce1 (one wave per chunk of kPartChunk force-pass
+// partials, many waves) and k_finalize (one workgroup over the chunk totals, then the step's
+// bookkeeping; mode 1 reduces k_props partials directly).  A single workgroup over all partials
+// took 26 us at 1.1 M atoms.
+constexpr int kFinalizeThreads = 256;
+template <bool M>
+__global__ __launch_bounds__(256) void k_reduce1(const Partial<typename Prec<M>::A> *__restrict__ part, int nParts,
+                                                 Partial<typename Prec<M>::A> *__restrict__ part2) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nChunks = (nParts + kPartChunk - 1) / kPartChunk;
+    if (j < nChunks) reduce_chunk(part + j * kPartChunk, min(kPartChunk, nParts - j * kPartChunk), part2 + j);
+}
+
 template <bool M>
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(DevParams p,
                                                                const Partial<typename Prec<M>::A> *__restrict__ part,
@@ -168,12 +178,23 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(DevParams p,
     finalize_body<typename Prec<M>::A>(p, part, nParts, st, records, recCap, mode);
 }
 
-// Batched: one workgroup per replica, the same reduction as k_finalize over the replica's partials.
+// Batched: the same two stages per replica (one workgroup per replica each), so a replica's
+// reduction order is that of the replica run alone.
+template <bool M>
+__global__ __launch_bounds__(256) void kb_reduce1(BatchArgs a) {
+    using A = typename Prec<M>::A;
+    const BatchRep &B = a.reps[blockIdx.x];
+    const Partial<A> *part = (const Partial<A> *)a.partials + B.partOff;
+    Partial<A> *part2 = (Partial<A> *)a.partials + B.part2Off;
+    for (int j = threadIdx.x >> 6; j < B.nChunks; j += 4)
+        reduce_chunk(part + j * kPartChunk, min(kPartChunk, B.nbForce - j * kPartChunk), part2 + j);
+}
+
 template <bool M>
 __global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a) {
     using A = typename Prec<M>::A;
     const BatchRep &B = a.reps[blockIdx.x];
-    finalize_body<A>(B.p, (const Partial<A> *)a.partials + B.partOff, B.nbForce, a.state + blockIdx.x,
+    finalize_body<A>(B.p, (const Partial<A> *)a.partials + B.part2Off, B.nChunks, a.state + blockIdx.x,
                      a.records + (size_t)blockIdx.x * a.recCap, a.recCap, 0);
 }
 
@@ -318,16 +339,28 @@ void launch_props(sgk_ctx *c) {
     profile_end(c, "props");
 }
 
+template <bool M>
+static void finalize_dispatch(sgk_ctx *c, int mode) {
+    using A = typename Prec<M>::A;
+    Partial<A> *part = (Partial<A> *)c->partials;
+    if (mode == 1) {
+        hipLaunchKernelGGL(k_finalize<M>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp, part, c->nbProps,
+                           c->state, c->records, 1, 1);
+        return;
+    }
+    const int nParts = c->nbM + c->nbC, nChunks = (nParts + kPartChunk - 1) / kPartChunk;
+    Partial<A> *part2 = part + c->part2Off;
+    hipLaunchKernelGGL(k_reduce1<M>, dim3((nChunks + 3) / 4), dim3(256), 0, c->stream, part, nParts, part2);
+    hipLaunchKernelGGL(k_finalize<M>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp, part2, nChunks, c->state,
+                       c->records, c->cfg.records_capacity, 0);
+}
+
 void launch_finalize(sgk_ctx *c, int mode) {
     profile_begin(c);
-    const int nParts = mode == 1 ? c->nbProps : c->nbM + c->nbC;
-    const int cap = mode == 1 ? 1 : c->cfg.records_capacity;
     if (c->mixed)
-        hipLaunchKernelGGL(k_finalize<true>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp,
-                           (const Partial<double> *)c->partials, nParts, c->state, c->records, cap, mode);
+        finalize_dispatch<true>(c, mode);
     else
-        hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(kFinalizeThreads), 0, c->stream, c->dp,
-                           (const Partial<float> *)c->partials, nParts, c->state, c->records, cap, mode);
+        finalize_dispatch<false>(c, mode);
     profile_end(c, "finalize");
 }
 
@@ -368,10 +401,13 @@ void launch_batch_lf1(sgk_batch *b) {
 }
 
 void launch_batch_finalize(sgk_batch *b) {
-    if (b->mixed)
+    if (b->mixed) {
+        hipLaunchKernelGGL(kb_reduce1<true>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
         hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
-    else
+    } else {
+        hipLaunchKernelGGL(kb_reduce1<false>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
         hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
+    }
 }
 
 void launch_batch_thermostat(sgk_batch *b) {
