@@ -149,6 +149,54 @@ __global__ __launch_bounds__(256) void kb_sortcells(BatchArgs a) {
 //   carbon atom: lj[q*nC + ic] = k<<8 | j   for metal k   (LJ partners with list position j)
 //                cl[ic*clCap + q] = k<<8 | j if r^2 < closeR2 (bond and angle candidates)
 //   keepFull:    fullNBL[j*nMol + i] = k, fullNN[i]      (reference layout)
+// Append neighbour k (at squared distance rr) to atom i's lists; nn, nljv, nclv are its running
+// counts (list position, metal partners, close partners).
+__device__ __forceinline__ void nl_emit(const DevParams &p, int i, bool metal, int ic, int k, float rr, int &nn,
+                                        int &nljv, int &nclv, int &err, int *__restrict__ nlM,
+                                        unsigned *__restrict__ lj, unsigned *__restrict__ cl,
+                                        int *__restrict__ fullNBL) {
+    if (nn < p.nlCap) {
+        if (p.keepFull) fullNBL[(size_t)nn * p.nMol + i] = k;
+        if (metal) {
+            nlM[(size_t)i * p.nlCap + nn] = k;
+        } else {
+            const unsigned ent = ((unsigned)k << 8) | (unsigned)nn;
+            if (k < p.nMolMe) {
+                if (nljv < p.ljCap)
+                    lj[(size_t)nljv * p.nC + ic] = ent;
+                else
+                    err |= SGK_FLAG_LJ_OVERFLOW;
+                ++nljv;
+            }
+            if (rr < p.closeR2) {
+                if (nclv < p.clCap)
+                    cl[(size_t)ic * p.clCap + nclv] = ent;
+                else
+                    err |= SGK_FLAG_CLOSE_OVERFLOW;
+                ++nclv;
+            }
+        }
+    } else {
+        err |= SGK_FLAG_NL_OVERFLOW;
+    }
+    ++nn;
+}
+
+// Counts of atom i's lists and its error flags.
+__device__ __forceinline__ void nl_finish(const DevParams &p, int i, bool metal, int ic, int nn, int nljv, int nclv,
+                                          int err, int *__restrict__ nnM, int *__restrict__ nlj,
+                                          int *__restrict__ ncl, int *__restrict__ fullNN, DevState *st) {
+    nn = min(nn, p.nlCap);
+    if (p.keepFull) fullNN[i] = nn;
+    if (metal) {
+        nnM[i] = nn;
+    } else {
+        nlj[ic] = min(nljv, p.ljCap);
+        ncl[ic] = min(nclv, p.clCap);
+    }
+    if (err) atomicOr(&st->errors, err);
+}
+
 template <bool M>
 __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
                                         const CellBufs &bufs, DevState *st, int force, int fromNext,
@@ -171,7 +219,7 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
         const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
         const int ccx = c - (ccz * p.cy + ccy) * p.cx;
         const float3 A = f3(pos[i]);
-        const float rrN = p.rrNebr, rrClose = p.closeR2;
+        const float rrN = p.rrNebr;
         float shx, shy, shz;
         int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
         int ncur = cnt[C];
@@ -201,31 +249,7 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
                     if (base + t >= n) continue;
                     const float rr = shifted_rr(A, B[t], shx, shy, shz);
                     if (!(rr < rrN && k[t] != i)) continue;
-                    if (nn < p.nlCap) {
-                        if (p.keepFull) fullNBL[(size_t)nn * p.nMol + i] = k[t];
-                        if (metal) {
-                            nlM[(size_t)i * p.nlCap + nn] = k[t];
-                        } else {
-                            const unsigned ent = ((unsigned)k[t] << 8) | (unsigned)nn;
-                            if (k[t] < p.nMolMe) {
-                                if (nljv < p.ljCap)
-                                    lj[(size_t)nljv * p.nC + ic] = ent;
-                                else
-                                    err |= SGK_FLAG_LJ_OVERFLOW;
-                                ++nljv;
-                            }
-                            if (rr < rrClose) {
-                                if (nclv < p.clCap)
-                                    cl[(size_t)ic * p.clCap + nclv] = ent;
-                                else
-                                    err |= SGK_FLAG_CLOSE_OVERFLOW;
-                                ++nclv;
-                            }
-                        }
-                    } else {
-                        err |= SGK_FLAG_NL_OVERFLOW;
-                    }
-                    ++nn;
+                    nl_emit(p, i, metal, ic, k[t], rr, nn, nljv, nclv, err, nlM, lj, cl, fullNBL);
                 }
             }
             C = Cn;
@@ -235,15 +259,152 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
             shz = nshz;
         }
     }
-    nn = min(nn, p.nlCap);
-    if (p.keepFull) fullNN[i] = nn;
-    if (metal) {
-        nnM[i] = nn;
-    } else {
-        nlj[ic] = min(nljv, p.ljCap);
-        ncl[ic] = min(nclv, p.clCap);
+    nl_finish(p, i, metal, ic, nn, nljv, nclv, err, nnM, nlj, ncl, fullNN, st);
+}
+
+// BuildNebrListK (SurfaceGrowth.cu:612-711) with the candidates staged in LDS.  The 256 atoms of a
+// workgroup are consecutive indices - spatial neighbours, since the generators emit atoms along
+// lattice rows - so the cells they can reach form a small box: their cells' bounding box plus one
+// cell on every side.  The workgroup loads every atom of that box once (counts, a block scan, then
+// positions and indices) and each thread walks its atom's 27 cells in LDS, in the reference order,
+// with the same FP32 distance test and the same cell-wrap shift as nl_body.  A workgroup whose box
+// exceeds the LDS budget (a metal cluster's rows, a row crossing the periodic boundary) falls back
+// to nl_body; the decision is uniform over the workgroup.
+constexpr int kNlMaxHalo = 1024;  // cells of the box
+constexpr int kNlMaxEnt = 2048;   // atoms in those cells
+
+// Exclusive prefix sum of a[0..n) (n <= 4 * blockDim.x, blockDim.x = 256) in place; a[n] = total.
+__device__ __forceinline__ void block_scan4(int *a, int n) {
+    __shared__ int wsum[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int v[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int h = 4 * t + q;
+        v[q] = h < n ? a[h] : 0;
+        sum += v[q];
     }
-    if (err) atomicOr(&st->errors, err);
+    int inc = sum;  // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += u;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int base = inc - sum;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int h = 4 * t + q;
+        if (h < n) a[h] = base;
+        base += v[q];
+    }
+    if (t == blockDim.x - 1) a[n] = base;
+    __syncthreads();
+}
+
+template <bool M>
+__device__ __forceinline__ void nlt_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                         const CellBufs &bufs, DevState *st, int force, int fromNext,
+                                         int *__restrict__ nlM, int *__restrict__ nnM, unsigned *__restrict__ lj,
+                                         int *__restrict__ nlj, unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                         int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    if (!force && !st->nebrNow) return;
+    __shared__ int sStart[kNlMaxHalo + 1];
+    __shared__ float4 sEnt[kNlMaxEnt];
+    __shared__ int sLo[3], sHi[3], sFall;
+    const int i = blk * blockDim.x + threadIdx.x;
+    const bool in = i < p.nMol;
+    const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
+    const int c = in ? bufs.cellOf[b][i] : -1;
+    int ccx = 0, ccy = 0, ccz = 0;
+    if (c >= 0) {
+        ccz = c / (p.cx * p.cy);
+        ccy = (c - ccz * p.cx * p.cy) / p.cx;
+        ccx = c - (ccz * p.cy + ccy) * p.cx;
+    }
+    if (threadIdx.x == 0) {
+        sLo[0] = sLo[1] = sLo[2] = 0x7fffffff;
+        sHi[0] = sHi[1] = sHi[2] = -1;
+    }
+    __syncthreads();
+    if (c >= 0) {
+        atomicMin(&sLo[0], ccx);
+        atomicMin(&sLo[1], ccy);
+        atomicMin(&sLo[2], ccz);
+        atomicMax(&sHi[0], ccx);
+        atomicMax(&sHi[1], ccy);
+        atomicMax(&sHi[2], ccz);
+    }
+    __syncthreads();
+    if (sHi[0] < 0) {  // no binned atom in the workgroup: empty lists
+        if (in) nl_finish(p, i, i < p.nMolMe, i - p.nMolMe, 0, 0, 0, 0, nnM, nlj, ncl, fullNN, st);
+        return;
+    }
+    const int x0 = sLo[0] - 1, y0 = sLo[1] - 1, z0 = sLo[2] - 1;  // box origin (grid coordinates)
+    const int BX = sHi[0] - sLo[0] + 3, BY = sHi[1] - sLo[1] + 3, BZ = sHi[2] - sLo[2] + 3;
+    const long long Hl = (long long)BX * BY * BZ;
+    const int *cnt = bufs.cnt[b];
+    const int *slots = bufs.slots[b];
+    // box cell h -> grid cell, with the reference's single wrap per axis (stencil_cell)
+    auto grid_cell = [&](int h) {
+        const int hz = h / (BX * BY), hy = (h / BX) % BY, hx = h % BX;
+        int gx = x0 + hx, gy = y0 + hy, gz = z0 + hz;
+        gx = gx >= p.cx ? 0 : (gx < 0 ? p.cx - 1 : gx);
+        gy = gy >= p.cy ? 0 : (gy < 0 ? p.cy - 1 : gy);
+        gz = gz >= p.cz ? 0 : (gz < 0 ? p.cz - 1 : gz);
+        return (gz * p.cy + gy) * p.cx + gx;
+    };
+    const int H = Hl <= kNlMaxHalo ? (int)Hl : 0;
+    if (threadIdx.x == 0) sFall = Hl > kNlMaxHalo;
+    for (int h = threadIdx.x; h < H; h += blockDim.x) sStart[h] = min(cnt[grid_cell(h)], p.cellCap);
+    __syncthreads();
+    if (!sFall) {
+        block_scan4(sStart, H);
+        if (threadIdx.x == 0) sFall = sStart[H] > kNlMaxEnt;
+        __syncthreads();
+    }
+    if (sFall) {  // uniform: the whole workgroup walks the cells in global memory
+        nl_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
+        return;
+    }
+    const int E = sStart[H];
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        int lo = 0, hi = H;  // last box cell whose start is <= e
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sStart[mid] <= e) lo = mid; else hi = mid;
+        }
+        const size_t g = (size_t)grid_cell(lo) * p.cellCap + (e - sStart[lo]);
+        float4 q = bufs.cellPos[g];
+        q.w = __int_as_float(slots[g]);
+        sEnt[e] = q;
+    }
+    __syncthreads();
+    if (!in) return;
+    const bool metal = i < p.nMolMe;
+    const int ic = i - p.nMolMe;
+    int nn = 0, nljv = 0, nclv = 0, err = 0;
+    if (c >= 0) {
+        const float3 A = f3(pos[i]);
+        const float rrN = p.rrNebr;
+        for (int o = 0; o < 27; ++o) {
+            float shx, shy, shz;
+            (void)stencil_cell<float>(p, ccx, ccy, ccz, o, shx, shy, shz);
+            const int h = (ccx + off_comp(o / 9) - x0) + BX * ((ccy + off_comp((o / 3) % 3) - y0) +
+                                                               BY * (ccz + off_comp(o % 3) - z0));
+            const int e1 = sStart[h + 1];
+            for (int e = sStart[h]; e < e1; ++e) {
+                const float4 q = sEnt[e];
+                const int k = __float_as_int(q.w);
+                const float rr = shifted_rr(A, make_float3(q.x, q.y, q.z), shx, shy, shz);
+                if (!(rr < rrN && k != i)) continue;
+                nl_emit(p, i, metal, ic, k, rr, nn, nljv, nclv, err, nlM, lj, cl, fullNBL);
+            }
+        }
+    }
+    nl_finish(p, i, metal, ic, nn, nljv, nclv, err, nnM, nlj, ncl, fullNN, st);
 }
 
 // BuildNebrListK (SurfaceGrowth.cu:612-711), one wave per occupied cell (the wave of the cell's
@@ -414,6 +575,18 @@ __global__ __launch_bounds__(256) void k_nl(DevParams p, const typename Prec<M>:
         nl_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
 }
 
+// The LDS-staged scheme in a kernel of its own (its 36 KB of LDS would limit the others' occupancy).
+template <bool M>
+__global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                             CellBufs bufs, DevState *st, int force,
+                                             int fromNext, int *__restrict__ nlM, int *__restrict__ nnM,
+                                             unsigned *__restrict__ lj, int *__restrict__ nlj,
+                                             unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                             int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    const int blk = (p.xcdMap & 1) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    nlt_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
+}
+
 // Batched: the step path (lists built from the new cell buffer before the commit).
 template <bool M>
 __global__ __launch_bounds__(256) void kb_nl(BatchArgs a) {
@@ -428,6 +601,16 @@ __global__ __launch_bounds__(256) void kb_nl(BatchArgs a) {
         nl_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
                    a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap,
                    a.nlj + B.carbonOff, a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void kb_nlt(BatchArgs a) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const DevParams &p = B.p;
+    nlt_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
+                a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff,
+                a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
 }
 
 // Make the freshly built buffer current (API path; the step path does this in its reduction).
@@ -532,14 +715,16 @@ void launch_nl(sgk_ctx *c, bool force) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMol, 256);
     const int fromNext = force ? 0 : 1;  // step path: build from the new buffer before the commit
-    if (c->mixed)
-        hipLaunchKernelGGL(k_nl<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (const double4 *)c->pos, cell_bufs(c),
-                           c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
-                           c->fullNN, c->fullNBL);
-    else
-        hipLaunchKernelGGL(k_nl<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (const float4 *)c->pos, cell_bufs(c),
-                           c->state, (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl,
-                           c->fullNN, c->fullNBL);
+    const bool tiled = c->dp.nlScheme == 2;
+#define SGK_NL(K, MM, P4)                                                                                         \
+    hipLaunchKernelGGL(K<MM>, dim3(g), dim3(256), 0, c->stream, c->dp, (const P4 *)c->pos, cell_bufs(c), c->state, \
+                       (int)force, fromNext, c->nlM, c->nnM, c->lj, c->nlj, c->cl, c->ncl, c->fullNN, c->fullNBL)
+    if (c->mixed) {
+        if (tiled) SGK_NL(k_nlt, true, double4); else SGK_NL(k_nl, true, double4);
+    } else {
+        if (tiled) SGK_NL(k_nlt, false, float4); else SGK_NL(k_nl, false, float4);
+    }
+#undef SGK_NL
     profile_end(c, "nl");
 }
 
@@ -551,10 +736,18 @@ void launch_batch_sortcells(sgk_batch *b) {
 }
 
 void launch_batch_nl(sgk_batch *b) {
-    if (b->mixed)
-        hipLaunchKernelGGL(kb_nl<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
-    else
-        hipLaunchKernelGGL(kb_nl<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    const bool tiled = b->cfg.nl_scheme == 2;
+    if (b->mixed) {
+        if (tiled)
+            hipLaunchKernelGGL(kb_nlt<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+        else
+            hipLaunchKernelGGL(kb_nl<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    } else {
+        if (tiled)
+            hipLaunchKernelGGL(kb_nlt<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+        else
+            hipLaunchKernelGGL(kb_nl<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+    }
 }
 
 void launch_batch_rdf(sgk_batch *b) {

@@ -36,18 +36,20 @@ def test_batch_rejects_replicas_with_different_potentials():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
-def test_batch_equals_replicas_run_alone(gpu, prec):
+@pytest.mark.parametrize("nl_scheme", [2, 0])
+def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
     systems = replicas()
     steps = 40
-    print(f"[batch] {prec}: {len(systems)} replicas", flush=True)
-    with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0) as bt:
+    print(f"[batch] {prec}, list scheme {nl_scheme}: {len(systems)} replicas", flush=True)
+    with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0,
+                  nl_scheme=nl_scheme) as bt:
         for k, s in enumerate(systems):
             bt.upload(k, s.r, s.v, s.a)
             bt.set_run_state(k, 0, 1, 0.0)
         recs = bt.run(steps)
         out = [bt.download(k) for k in range(len(systems))]
     for k, s in enumerate(systems):
-        with sg.Engine(s.params, precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0) as eng:
+        with sg.Engine(s.params, precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0, nl_scheme=nl_scheme) as eng:
             eng.upload(s.r, s.v, s.a)
             eng.set_run_state(0, 1, 0.0)
             ref = eng.run(steps)

@@ -42,32 +42,48 @@ def force_error(acc, ref):
     return np.abs(acc - ref).max() / rms
 
 
-@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
-@pytest.mark.parametrize("prec", ["fp32", "mixed"])
-@pytest.mark.parametrize("scheme", [1, 0])
-def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
-    """Both list schemes (one wave per cell, one thread per atom) give the reference list."""
-    s = system(name)
-    p = s.params
+def check_lists(p, prec, scheme, r):
     o = oracle.Oracle(p, "f32")
-    print(f"[parity] cells/NL {name} {prec} scheme {scheme}", flush=True)
     with sg.Engine(p, precision=prec, keep_full_nl=True, nl_scheme=scheme) as eng:
-        eng.upload(s.r, s.v, s.a)
+        eng.upload(r)
         eng.bin()
         cell_of, cnt, slots = eng.get_cells()
-        err, ocell, ocnt, oslots = o.bin(s.r)
+        err, ocell, ocnt, oslots = o.bin(r)
         assert err == 0
         assert np.array_equal(cell_of, ocell)
         assert np.array_equal(cnt, ocnt)
         assert np.array_equal(slots, oslots)
         eng.build_nebr()
         nn, nbl = eng.get_nebr()
-        err, onn, onbl = o.nebr(s.r, ocnt, oslots)
+        err, onn, onbl = o.nebr(r, ocnt, oslots)
         assert err == 0
         assert np.array_equal(nn, onn)
         assert np.array_equal(nbl, onbl)
         if p.nMol > p.nMolMe:
-            check_carbon_lists(eng, p, s.r.astype(np.float32), onn, onbl)
+            check_carbon_lists(eng, p, np.asarray(r, np.float32), onn, onbl)
+
+
+@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("scheme", [2, 1, 0])
+def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
+    """Every list scheme (LDS-staged per workgroup, one wave per cell, one thread per atom) gives the
+    reference list."""
+    s = system(name)
+    print(f"[parity] cells/NL {name} {prec} scheme {scheme}", flush=True)
+    check_lists(s.params, prec, scheme, s.r)
+
+
+@pytest.mark.parametrize("name", ["icosa", "medium"])
+@pytest.mark.parametrize("scheme", [2, 0])
+def test_neighbour_list_bit_exact_thermalised(gpu, name, scheme):
+    """On a thermalised state the sheet straddles a cell boundary in z and the cluster has moved:
+    more occupied cells per workgroup box, and the metal workgroups exceed the LDS budget, so the
+    staged scheme runs both its LDS path and its fallback."""
+    s = system(name)
+    r, _, _ = thermalised(name)
+    print(f"[parity] NL thermalised {name} scheme {scheme}", flush=True)
+    check_lists(s.params, "mixed", scheme, r)
 
 
 def check_carbon_lists(eng, p, r, nn, nbl):
