@@ -238,26 +238,33 @@ __global__ void k_adh_fin(const double *__restrict__ part, int n, double *__rest
     out[recIndex ? st->rec - 1 : 0] = s;
 }
 
-// FP64 density, F' and F'' of every metal atom (EamComputeRhoK, SurfaceGrowth.cu:716-773).
+// FP64 density, F' and F'' of every metal atom (EamComputeRhoK, SurfaceGrowth.cu:716-773); G lanes
+// per atom share its neighbour loop, as in the forward density pass.
+template <int G>
 __global__ __launch_bounds__(kBs) void k_adj_rho(DevParams p, const double4 *__restrict__ pos,
                                                  const int *__restrict__ nlM, const int *__restrict__ nnM,
                                                  double *__restrict__ rho, double *__restrict__ dF,
                                                  double *__restrict__ ddF) {
-    const int i = blockIdx.x * kBs + threadIdx.x;
-    if (i >= p.nMolMe) return;
-    const double4 A = pos[i];
-    const int *row = nlM + (size_t)i * p.nlCap;
-    const int nn = nnM[i];
+    const int gt = blockIdx.x * kBs + threadIdx.x, i = gt / G, l = gt % G;
+    const bool active = i < p.nMolMe;
     double sum = 0.0;
-    for (int j = 0; j < nn; ++j) {
-        const int k = row[j];
-        if (k >= p.nMolMe) continue;
-        double d[3];
-        const double rr = dispd(A, pos[k], p, d);
-        if (rr < (double)p.rrCutEam) sum += eam_d(p, sqrt(rr)).f;
+    if (active) {
+        const double4 A = pos[i];
+        const int *row = nlM + (size_t)i * p.nlCap;
+        const int nn = nnM[i];
+        for (int j = l; j < nn; j += G) {
+            const int k = row[j];
+            if (k >= p.nMolMe) continue;
+            double d[3];
+            const double rr = dispd(A, pos[k], p, d);
+            if (rr < (double)p.rrCutEam) sum += eam_d(p, sqrt(rr)).f;
+        }
     }
-    rho[i] = sum;
-    embed_d(p, sum, dF[i], ddF[i]);
+    sum = group_sum<G>(sum);  // every lane of the wave takes part
+    if (active && l == 0) {
+        rho[i] = sum;
+        embed_d(p, sum, dF[i], ddF[i]);
+    }
 }
 
 struct AdjArgs {
@@ -268,8 +275,29 @@ struct AdjArgs {
     AdjDev d;
 };
 
-// Metal atom i: EAM pairs and LJ with carbon (SurfaceGrowth.cu:856-896).
-__device__ void metal_vjp(const DevParams &p, int i, const AdjArgs &a, double wE, double (&pb)[4]) {
+__device__ void metal_pairs(const DevParams &p, int i, int gl, int stride, const AdjArgs &a, double wE,
+                            double (&pb)[4], double (&gi)[3], double &rb);
+
+// Metal atom i: EAM pairs and LJ with carbon (SurfaceGrowth.cu:856-896).  G lanes per atom: lane
+// gl takes list entries gl, gl+G, ...; the atom's own derivative and density adjoint are summed over
+// its lanes (every lane of the wave calls this).
+template <int G>
+__device__ void metal_vjp(const DevParams &p, int i, int gl, bool active, const AdjArgs &a, double wE,
+                          double (&pb)[4]) {
+    const int N = p.nMol;
+    double gi[3] = {0.0, 0.0, 0.0}, rb = 0.0;
+    if (active) metal_pairs(p, i, gl, G, a, wE, pb, gi, rb);
+    for (int x = 0; x < 3; ++x) gi[x] = group_sum<G>(gi[x]);
+    rb = group_sum<G>(rb);
+    if (active && gl == 0) {
+        scatter(a.d.G, N, i, 1.0, gi);
+        if (rb != 0.0) atomicAdd(&a.d.rhoBar[i], rb);
+    }
+}
+
+// The pair loop of metal_vjp over list entries gl, gl+stride, ...
+__device__ void metal_pairs(const DevParams &p, int i, int gl, int stride, const AdjArgs &a, double wE,
+                            double (&pb)[4], double (&gi)[3], double &rb) {
     const int N = p.nMol;
     const double im = 1.0 / (double)p.massMe;
     const double l[3] = {a.d.Lam[i] * im, a.d.Lam[N + i] * im, a.d.Lam[2 * N + i] * im};
@@ -277,10 +305,9 @@ __device__ void metal_vjp(const DevParams &p, int i, const AdjArgs &a, double wE
     if (!hasL && wE == 0.0) return;
     const double4 A = a.pos[i];
     const double Fi1 = a.d.dF[i], Fi2 = a.d.ddF[i];
-    double gi[3] = {0.0, 0.0, 0.0}, rb = 0.0;
     const int *row = a.nlM + (size_t)i * p.nlCap;
     const int nn = a.nnM[i];
-    for (int j = 0; j < nn; ++j) {
+    for (int j = gl; j < nn; j += stride) {
         const int k = row[j];
         double d[3];
         const double rr = dispd(A, a.pos[k], p, d);
@@ -311,8 +338,6 @@ __device__ void metal_vjp(const DevParams &p, int i, const AdjArgs &a, double wE
         for (int x = 0; x < 3; ++x) gi[x] += g[x];
         scatter(a.d.G, N, k, -1.0, g);
     }
-    scatter(a.d.G, N, i, 1.0, gi);
-    if (rb != 0.0) atomicAdd(&a.d.rhoBar[i], rb);
 }
 
 // Carbon atom i: bonds, plane, angle terms and LJ with metal (SurfaceGrowth.cu:899-1062), the
@@ -420,12 +445,13 @@ __device__ void carbon_vjp(const DevParams &p, int ic, const AdjArgs &a, double 
         if (b < counti) scatter(a.d.G, N, kb[b], 1.0, gk[b]);
 }
 
-// blocks [0, nbM): metal atoms, [nbM, nbM + nbC): carbon atoms; one thread per atom.
+// blocks [0, nbM): metal atoms, G lanes each; [nbM, nbM + nbC): carbon atoms, one thread each.
+template <int G>
 __global__ __launch_bounds__(kBs) void k_adj_force(DevParams p, AdjArgs a, int nbM, double wE) {
     double pb[4] = {0.0, 0.0, 0.0, 0.0};
     if ((int)blockIdx.x < nbM) {
-        const int i = blockIdx.x * kBs + threadIdx.x;
-        if (i < p.nMolMe) metal_vjp(p, i, a, wE, pb);
+        const int gt = blockIdx.x * kBs + threadIdx.x, i = gt / G;
+        metal_vjp<G>(p, i, gt % G, i < p.nMolMe, a, wE, pb);
     } else {
         const int ic = (blockIdx.x - nbM) * kBs + threadIdx.x;
         if (ic < p.nC) carbon_vjp(p, ic, a, pb);
@@ -435,19 +461,20 @@ __global__ __launch_bounds__(kBs) void k_adj_force(DevParams p, AdjArgs a, int n
 
 // Density adjoint to positions: rho_i = sum_k f(r_ik), so d/dr_i += rhoBar_i f'(r) d/r and
 // d/dr_k -= the same; and rho = s_rho rho0 gives pbar_density += rhoBar_i rho_i.
+template <int G>
 __global__ __launch_bounds__(kBs) void k_adj_rho2(DevParams p, AdjArgs a) {
-    const int i = blockIdx.x * kBs + threadIdx.x;
-    double p3 = 0.0;
-    if (i < p.nMolMe) {
+    const int gt = blockIdx.x * kBs + threadIdx.x, i = gt / G, l = gt % G;
+    const bool active = i < p.nMolMe;
+    const int N = p.nMol;
+    double p3 = 0.0, gi[3] = {0.0, 0.0, 0.0};
+    if (active) {
         const double rb = a.d.rhoBar[i];
-        p3 = rb * a.d.rho[i];
+        if (l == 0) p3 = rb * a.d.rho[i];
         if (rb != 0.0) {
-            const int N = p.nMol;
             const double4 A = a.pos[i];
             const int *row = a.nlM + (size_t)i * p.nlCap;
             const int nn = a.nnM[i];
-            double gi[3] = {0.0, 0.0, 0.0};
-            for (int j = 0; j < nn; ++j) {
+            for (int j = l; j < nn; j += G) {
                 const int k = row[j];
                 if (k >= p.nMolMe) continue;
                 double d[3];
@@ -459,9 +486,10 @@ __global__ __launch_bounds__(kBs) void k_adj_rho2(DevParams p, AdjArgs a) {
                 for (int x = 0; x < 3; ++x) gi[x] += g[x];
                 scatter(a.d.G, N, k, -1.0, g);
             }
-            scatter(a.d.G, N, i, 1.0, gi);
         }
     }
+    for (int x = 0; x < 3; ++x) gi[x] = group_sum<G>(gi[x]);
+    if (active && l == 0) scatter(a.d.G, N, i, 1.0, gi);
     add_pbar(a.d.pbar, 0.0, 0.0, 0.0, p3);
 }
 
@@ -625,21 +653,41 @@ void launch_adh_obs(sgk_ctx *c, double *part, double *out, int recIndex) {
     hipLaunchKernelGGL(k_adh_fin, dim3(1), dim3(64), 0, c->stream, part, (int)g, out, c->state, recIndex);
 }
 
+// Lanes per metal atom of the metal kernels: the context's forward choice (sgk_config.metal_lanes).
+#define SGK_ADJ_G(call)            \
+    switch (c->metalLanes) {       \
+        case 1: call(1); break;    \
+        case 2: call(2); break;    \
+        case 4: call(4); break;    \
+        case 16: call(16); break;  \
+        default: call(8); break;   \
+    }
+
 void launch_adj_rho(sgk_ctx *c, const AdjDev &d) {
     if (c->dp.nMolMe == 0) return;
-    hipLaunchKernelGGL(k_adj_rho, dim3(blocks(c->dp.nMolMe)), dim3(kBs), 0, c->stream, c->dp,
-                       (const double4 *)c->pos, c->nlM, c->nnM, d.rho, d.dF, d.ddF);
+    const unsigned g = blocks((long long)c->dp.nMolMe * c->metalLanes);
+#define K(G)                                                                                                 \
+    hipLaunchKernelGGL(k_adj_rho<G>, dim3(g), dim3(kBs), 0, c->stream, c->dp, (const double4 *)c->pos, c->nlM, \
+                       c->nnM, d.rho, d.dF, d.ddF)
+    SGK_ADJ_G(K)
+#undef K
 }
 
 void launch_adj_force(sgk_ctx *c, const AdjDev &d, double wE) {
-    const int nbM = (int)blocks(c->dp.nMolMe), nbC = (int)blocks(c->dp.nC);
+    const int nbM = (int)blocks((long long)c->dp.nMolMe * c->metalLanes), nbC = (int)blocks(c->dp.nC);
     if (nbM + nbC == 0) return;
-    hipLaunchKernelGGL(k_adj_force, dim3(nbM + nbC), dim3(kBs), 0, c->stream, c->dp, adj_args(c, d), nbM, wE);
+#define K(G) \
+    hipLaunchKernelGGL(k_adj_force<G>, dim3(nbM + nbC), dim3(kBs), 0, c->stream, c->dp, adj_args(c, d), nbM, wE)
+    SGK_ADJ_G(K)
+#undef K
 }
 
 void launch_adj_rho2(sgk_ctx *c, const AdjDev &d) {
     if (c->dp.nMolMe == 0) return;
-    hipLaunchKernelGGL(k_adj_rho2, dim3(blocks(c->dp.nMolMe)), dim3(kBs), 0, c->stream, c->dp, adj_args(c, d));
+    const unsigned g = blocks((long long)c->dp.nMolMe * c->metalLanes);
+#define K(G) hipLaunchKernelGGL(k_adj_rho2<G>, dim3(g), dim3(kBs), 0, c->stream, c->dp, adj_args(c, d))
+    SGK_ADJ_G(K)
+#undef K
 }
 
 void launch_adj_therm(sgk_ctx *c, const AdjDev &d, const double *velPre, int step) {
