@@ -224,9 +224,9 @@ void sgk_default_config(sgk_config *cfg) {
     cfg->records_capacity = 1024;
     cfg->use_graphs = 1;
     cfg->pair_fp64 = 1;  // mixed form: FP64 pair arithmetic (measured cost nil, error < 1e-5)
-    cfg->xcd_map = 6;    // density and force passes
+    cfg->xcd_map = 0;    // remap measured 1-3% slower per step on MI355X; kept as an option
     cfg->bin_group = 1;
-    cfg->nl_scheme = 0;  // per atom: 1.35 ms per rebuild at 1.1 M atoms; per cell 3.4 ms
+    cfg->nl_scheme = 0;  // per atom: 0.32-0.45 ms per rebuild at 1.1 M atoms; per cell ~3 ms
 }
 
 int sgk_device_count(void) {
@@ -293,8 +293,8 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const DevParams &p = c->dp;
     // lanes per metal atom: enough waves to cover the machine for small clusters
     int G = cfg.metal_lanes;
-    // measured on MI355X: 2 lanes for 100k metal atoms, 4 for 4k, more for small clusters
-    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 2 : p.nMolMe >= 2048 ? 4 : 16;
+    // measured on MI355X (sgk_autotune): 4 lanes for 100k metal atoms, 8 for 4k, 16 for small clusters
+    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 4 : p.nMolMe >= 2048 ? 8 : 16;
     c->metalLanes = G;
     c->block = 256;
     // metal ranges padded to a multiple of 8 workgroups so the carbon range starts on XCD 0

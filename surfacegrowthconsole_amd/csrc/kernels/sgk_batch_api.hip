@@ -184,7 +184,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
     int G = cfg.metal_lanes;
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) {
         const long long avg = sumMe / nrep;
-        G = avg >= 65536 ? 2 : avg >= 2048 ? 4 : 16;
+        G = avg >= 65536 ? 4 : avg >= 2048 ? 8 : 16;
     }
     b->metalLanes = G;
     b->cfg.metal_lanes = G;

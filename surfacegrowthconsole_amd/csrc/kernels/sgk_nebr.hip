@@ -84,6 +84,45 @@ __global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
     bin_atom(p, i, f3(pos[i]), bufs, st);
 }
 
+// Sort the cells owned by the lanes in `own` (cell index cn, count ocnt in the owner lane), 64 / W
+// cells per pass: lanes [g W, g W + W) sort the g-th cell of the pass with a bitonic network in
+// registers, then store the sorted slots and the cell-ordered float positions.  Wave-collective.
+template <int W, class P4>
+__device__ __forceinline__ void sort_cells(const DevParams &p, const CellBufs &bufs, int b, const P4 *__restrict__ pos,
+                                           int cn, int ocnt, unsigned long long own) {
+    constexpr int kPer = 64 / W;
+    const int lane = threadIdx.x & 63, g = lane / W, idx = lane % W;
+    while (own) {
+        int src = -1;  // owner lane of this lane's cell in the pass
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int l = own ? __ffsll((long long)own) - 1 : -1;
+            if (own) own &= own - 1ull;
+            if (g == q) src = l;
+        }
+        // every lane takes part in both exchanges (a lane that skipped one would supply no data)
+        const int c = __shfl(cn, src < 0 ? 0 : src, 64);
+        const int cntSrc = __shfl(ocnt, src < 0 ? 0 : src, 64);
+        const int cnt = src < 0 ? 0 : cntSrc;
+        int *s = bufs.slots[b] + (size_t)(src < 0 ? 0 : c) * p.cellCap;
+        int v = idx < cnt ? s[idx] : 0x7fffffff;
+#pragma unroll
+        for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int w = __shfl_xor(v, j, 64);
+                const bool lower = (idx & j) == 0, up = (idx & k) == 0;
+                v = (lower == up) ? min(v, w) : max(v, w);
+            }
+        }
+        if (idx < cnt) {
+            s[idx] = v;
+            const float3 q = f3(pos[v]);
+            bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(q.x, q.y, q.z, 0.f);
+        }
+    }
+}
+
 // Sort each occupied cell of the new buffer (<= 64 atoms, one per lane) ascending and store its
 // atoms' float positions in slot order; clear the counts of the outgoing buffer.  One thread per
 // atom; the owner of a cell is the atom holding slot 0, and the owner's wave sorts the cell.
@@ -92,7 +131,6 @@ __device__ __forceinline__ void sortcells_body(const DevParams &p, int blk, cons
                                                const CellBufs &bufs, DevState *st, int force) {
     if (!force && !st->nebrNow) return;
     const int b = st->cellbuf ^ 1, old = st->cellbuf;
-    const int lane = threadIdx.x & 63;
     const int n = blk * blockDim.x + threadIdx.x;
     const bool in = n < p.nMol;  // every lane stays for the wave-wide sort
     int cn = -1;
@@ -101,31 +139,13 @@ __device__ __forceinline__ void sortcells_body(const DevParams &p, int blk, cons
         if (co >= 0) bufs.cnt[old][co] = 0;
         cn = bufs.cellOf[b][n];
     }
-    unsigned long long todo = __ballot(in && cn >= 0 && bufs.slotOf[n] == 0);
-    while (todo) {
-        const int l = __ffsll((long long)todo) - 1;
-        todo &= todo - 1ull;
-        const int c = __shfl(cn, l, 64);
-        const int cnt = min(bufs.cnt[b][c], p.cellCap);
-        int *s = bufs.slots[b] + (size_t)c * p.cellCap;
-        int v = lane < cnt ? s[lane] : 0x7fffffff;
-        if (cnt > 1) {
-#pragma unroll
-            for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    const int w = __shfl_xor(v, j, 64);
-                    const bool lower = (lane & j) == 0, up = (lane & k) == 0;
-                    v = (lower == up) ? min(v, w) : max(v, w);
-                }
-            }
-        }
-        if (lane < cnt) {
-            s[lane] = v;
-            const float3 q = f3(pos[v]);
-            bufs.cellPos[(size_t)c * p.cellCap + lane] = make_float4(q.x, q.y, q.z, 0.f);
-        }
-    }
+    const bool owner = in && cn >= 0 && bufs.slotOf[n] == 0;
+    const int ocnt = owner ? min(bufs.cnt[b][cn], p.cellCap) : 0;
+    // Cells of up to 16 atoms (most of them) are sorted four at a time, 16 lanes each; up to 32, two
+    // at a time; larger ones with the whole wave.
+    sort_cells<16>(p, bufs, b, pos, cn, ocnt, __ballot(owner && ocnt <= 16));
+    sort_cells<32>(p, bufs, b, pos, cn, ocnt, __ballot(owner && ocnt > 16 && ocnt <= 32));
+    sort_cells<64>(p, bufs, b, pos, cn, ocnt, __ballot(owner && ocnt > 32));
 }
 
 template <bool M>

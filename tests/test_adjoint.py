@@ -147,3 +147,27 @@ def test_adjoint_rejects_what_it_does_not_differentiate(gpu, tiny):
         eng.upload(s.r.astype(np.float64), s.v.astype(np.float64))
         with pytest.raises(RuntimeError, match="no forward window"):
             eng.adjoint_backward([1.0, 0.0])
+
+
+def test_bulk_metal_height_gradients_closed_form(gpu):
+    """Bulk Ni (regime 0, no thermostat): forces sum to zero pair by pair, so the metal centre of mass
+    moves ballistically and J_h = mean z_0 + W dt mean vz_0.  Its gradients are closed forms:
+    d J_h/d v0_z = W dt / nMe, d J_h/d r0_z = 1 / nMe, zero for x, y and for every parameter - which
+    checks the transpose of the leapfrog and of the momentum-conserving EAM pass without an oracle."""
+    s = sg.make_system(regime=0, material=3, ux=4, uy=4, uz=4, temperature=600.0, velocities="maxwell", seed=2)
+    p = s.params
+    W = 30
+    print(f"[adjoint] bulk {p.nMol} atoms, window {W}", flush=True)
+    with sg.Engine(p, precision="mixed") as eng:
+        eng.upload(s.r.astype(np.float64), s.v.astype(np.float64))
+        obs = eng.adjoint_forward(W, 10)
+        g, dr0, dv0 = eng.adjoint_backward([0.0, 1.0], initial_state=True)
+        info = eng.adjoint_info()
+    assert info["mismatches"] == 0
+    assert obs["adhesion"] == 0.0
+    n = p.nMolMe
+    dt = float(np.float32(p.deltaT))
+    assert np.abs(g).max() < 1e-12
+    assert np.allclose(dv0[:, 2], W * dt / n, rtol=1e-9, atol=0)
+    assert np.allclose(dr0[:, 2], 1.0 / n, rtol=1e-9, atol=0)
+    assert np.abs(dv0[:, :2]).max() < 1e-12 * W * dt / n and np.abs(dr0[:, :2]).max() < 1e-12 / n
