@@ -63,7 +63,9 @@ typedef struct sgk_config {
     int xcd_map;          /* launch shape: bit 0 neighbour list, bit 1 density/near lists, bit 2 force
                              pass - give each of the 8 XCDs a contiguous 1/8 of the atoms */
     int bin_group;        /* binning: one atomicAdd per distinct cell of a wave instead of per atom */
-    int nl_scheme;        /* neighbour list: 0 one thread per atom, 1 one wave per occupied cell */
+    int nl_scheme;        /* neighbour list: 0 one thread per atom, 1 one wave per occupied cell,
+                             2 LDS-staged per workgroup, 3 lanes per metal atom and carbons in cell
+                             order (sgk_build_nl_order); every scheme gives the same lists */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with
@@ -119,6 +121,11 @@ int sgk_bin(sgk_ctx *ctx);
 int sgk_get_cells(sgk_ctx *ctx, int *cellOf, int *cellCount, int *cellSlots);
 /* Neighbour list (BuildNebrListK, SurfaceGrowth.cu:612-711) from the current cells. */
 int sgk_build_nebr(sgk_ctx *ctx);
+/* nl_scheme 3: rebuild now the order in which the list build visits carbon atoms (by the cell of
+ * their current position).  sgk_run rebuilds it when it is missing or 100 steps old; the order
+ * changes which lane builds which list, never the lists.  No counterpart in the reference (its list
+ * build has one thread per atom in index order).  A no-op for the other schemes. */
+int sgk_build_nl_order(sgk_ctx *ctx);
 /* Reference layout NN[nMol], NBL[iNebrMax*nMol] column-major, -1 filled; needs keep_full_nl. */
 int sgk_get_nebr(sgk_ctx *ctx, int *NN, int *NBL);
 /* Per-carbon lists the force kernel uses: metal neighbours and close neighbours, each entry

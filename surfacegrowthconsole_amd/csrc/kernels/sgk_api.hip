@@ -226,7 +226,7 @@ void sgk_default_config(sgk_config *cfg) {
     cfg->pair_fp64 = 1;  // mixed form: FP64 pair arithmetic (measured cost nil, error < 1e-5)
     cfg->xcd_map = 0;    // remap measured 1-3% slower per step on MI355X; kept as an option
     cfg->bin_group = 1;
-    cfg->nl_scheme = 0;  // per atom: 0.32-0.45 ms per rebuild at 1.1 M atoms; per cell ~3 ms
+    cfg->nl_scheme = 3;  // 1.1 M atoms, thermalised: 0.35-0.37 ms per rebuild; per atom (0) 0.42 ms
 }
 
 int sgk_device_count(void) {
@@ -330,6 +330,13 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     SGK_TRY(dalloc(c, &c->ncl, nC, true));
     SGK_TRY(dalloc(c, &c->bondC, nC, true));
     SGK_TRY(dalloc(c, &c->bondA, nC, true));
+    if (p.nlScheme == 3 && nC > 0) {
+        SGK_TRY(dalloc(c, &c->nlOrder, nMol, false));
+        SGK_TRY(dalloc(c, &c->ordCnt, (size_t)p.ncell + 1, false));
+        SGK_TRY(dalloc(c, &c->ordKey, nMol, false));
+        SGK_TRY(dalloc(c, &c->ordSlot, nMol, false));
+        SGK_TRY(dalloc(c, &c->ordBlock, ((size_t)p.ncell + 1 + 4095) / 4096, false));
+    }
     if (p.keepFull) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
         SGK_TRY(dalloc(c, &c->fullNBL, nMol * (size_t)p.nlCap, false));
@@ -377,7 +384,8 @@ int sgk_destroy(sgk_ctx *c) {
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
                     c->cnt[0], c->cnt[1], c->slots[0], c->slots[1], c->slotOf, c->cellPos, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
-                    c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist};
+                    c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist,
+                    c->nlOrder, c->ordCnt, c->ordKey, c->ordSlot, c->ordBlock};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->hostState) (void)hipHostFree(c->hostState);
@@ -407,6 +415,7 @@ int sgk_upload(sgk_ctx *c, const sg_float4 *r, const sg_float3 *v, const sg_floa
         }
         c->haveList = false;
         c->haveCells = false;
+        c->orderValid = false;
     }
     if (v) {
         if (c->mixed) {
@@ -462,6 +471,7 @@ int sgk_upload_f64(sgk_ctx *c, const double *r, const double *v, const double *a
         }
         c->haveList = false;
         c->haveCells = false;
+        c->orderValid = false;
     }
     if (v) {
         if (c->mixed) {
@@ -596,6 +606,13 @@ int sgk_build_nebr(sgk_ctx *c) {
     int rc = check_flags(c);
     if (rc == SGK_OK) c->haveList = true;
     return rc;
+}
+
+int sgk_build_nl_order(sgk_ctx *c) {
+    if (!c) return SGK_E_ARG;
+    sgk::build_nl_order(c);
+    SGK_HIP(c, hipGetLastError());
+    return SGK_OK;
 }
 
 int sgk_get_nebr(sgk_ctx *c, int *NN, int *NBL) {
@@ -801,6 +818,9 @@ int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, do
     if (adh && (!c->adj || c->adj->every < nsteps)) return fail(c, SGK_E_STATE, "no adhesion-energy buffer for the run");
     const std::string flags = sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, adh != nullptr);
     SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    // the carbon list-build order of nl_scheme 3 follows the cells of the current positions; it decides
+    // only which lane builds which atom's lists, so it is refreshed now and then, outside the graph
+    if (c->nlOrder && (!c->orderValid || c->orderAge >= sgk::kNlOrderSteps)) sgk::build_nl_order(c);
     if (c->cfg.use_graphs && !c->profiling) {
         auto it = c->graphs.find(flags);
         if (it == c->graphs.end()) {
@@ -842,6 +862,7 @@ int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, do
     c->lastRunMs = ms;
     if (c->profiling) flush_profile(c);
     c->hostStep += nsteps;
+    c->orderAge += nsteps;
     c->haveList = true;
     c->haveCells = false;  // cells are those of the last rebuild, not of the current positions
     return check_flags(c);

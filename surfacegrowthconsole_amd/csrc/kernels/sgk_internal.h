@@ -52,6 +52,11 @@ struct sgk_ctx {
     int *nlj = nullptr, *ncl = nullptr;
     uint4 *bondC = nullptr, *bondA = nullptr;          // carbon: per-step near lists (k_rho)
     int *fullNN = nullptr, *fullNBL = nullptr;          // reference layout (keep_full_nl)
+    // order in which nl_scheme 3 builds the carbon lists (carbons by cell, then metal atoms) and the
+    // counting-sort scratch that makes it (sgk_nebr.hip)
+    int *nlOrder = nullptr, *ordCnt = nullptr, *ordKey = nullptr, *ordSlot = nullptr, *ordBlock = nullptr;
+    bool orderValid = false;
+    int orderAge = 0;                                   // steps run since the order was built
     // reductions and run state
     void *partials = nullptr;
     sgk::DevState *state = nullptr;
@@ -158,6 +163,8 @@ void launch_batch_nl(sgk_batch *b);
 void launch_batch_rdf(sgk_batch *b);
 void launch_sortcells(sgk_ctx *c, bool force);
 void launch_nl(sgk_ctx *c, bool force);
+void build_nl_order(sgk_ctx *c);
+constexpr int kNlOrderSteps = 100;  // age (steps) at which sgk_run rebuilds the nl_scheme 3 carbon order
 void launch_commit_rebuild(sgk_ctx *c);
 void launch_rdf(sgk_ctx *c);
 // sgk_force.hip

@@ -5,9 +5,10 @@
 //               (bin_atom; in the step loop it runs inside the leapfrog kernel).
 //   k_sortcells per atom: the atom that took slot 0 of its cell has its wave sort that cell's
 //               slots ascending (bitonic, in registers) and write the cell-ordered float
-//               positions; every atom clears its cell's count in cur for the rebuild after this
-//               one.  No global counter is involved.
-//   k_nl        one thread per atom: the reference neighbour order from the sorted cells.
+//               positions (padded with a far-away point to a multiple of 8); every atom clears its
+//               cell's count in cur for the rebuild after this one.  No global counter is involved.
+//   k_nlc       the lists in the reference order from the sorted cells: lanes per metal atom, one
+//               lane per carbon in cell order (nl_scheme 3, the default; schemes 0-2: k_nl, k_nlt).
 //   commit      cellbuf = nxt (the step reduction, or k_commit).
 // The API path (sgk_bin) bins with k_bin.  In the device-resident loop the kernels return at once
 // unless state.nebrNow is set, so a captured step needs no host decision.
@@ -84,6 +85,10 @@ __global__ void k_bin(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
     bin_atom(p, i, f3(pos[i]), bufs, st);
 }
 
+// Coordinate of the padding entries of the cell-ordered positions: its squared distance to any atom
+// (~3e36) is finite in FP32 and beyond every cutoff.
+constexpr float kFarAway = 1.0e18f;
+
 // Sort the cells owned by the lanes in `own` (cell index cn, count ocnt in the owner lane), 64 / W
 // cells per pass: lanes [g W, g W + W) sort the g-th cell of the pass with a bitonic network in
 // registers, then store the sorted slots and the cell-ordered float positions.  Wave-collective.
@@ -119,6 +124,10 @@ __device__ __forceinline__ void sort_cells(const DevParams &p, const CellBufs &b
             s[idx] = v;
             const float3 q = f3(pos[v]);
             bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(q.x, q.y, q.z, 0.f);
+        } else if (idx < ((cnt + 7) & ~7)) {
+            // padding up to the next multiple of 8: a position no list radius reaches, so a walk in
+            // chunks of 8 needs no per-candidate bound test
+            bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(kFarAway, kFarAway, kFarAway, 0.f);
         }
     }
 }
@@ -607,6 +616,313 @@ __global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>
     nlt_body<M>(p, blk, pos, bufs, st, force, fromNext, nlM, nnM, lj, nlj, cl, ncl, fullNN, fullNBL);
 }
 
+// ---- cell-ordered scheme (nl_scheme 3) ---------------------------------------------------------
+// BuildNebrListK (SurfaceGrowth.cu:612-711) in two parts of one launch; output as nl_body, bit for
+// bit.  The per-atom walk above is bound at once by the L1 (a wave's 64 atoms lie in ~20 different
+// cells, so each gather touches ~20 cache lines), by VALU issue and by the scalar unit, and its metal
+// atoms - ~480 candidates each, one lane per atom - finish last.  Here:
+//   metal part  (blocks [0, nbM)): kNlMetalLanes lanes per metal atom, atoms in index order.  The
+//               lanes of an atom take two candidates each of every chunk of 8 in the walk order; a
+//               ballot over the group gives each hit its list position.
+//   carbon part (blocks [nbM, nbM + nbC)): one lane per carbon, carbons in the order of their cells
+//               (nlOrder, built by build_nl_order), so the lanes of a wave hold a few adjacent cells
+//               that share their neighbour cells and their candidate loads hit the same cache lines.
+//               The walk has no per-candidate bound test (positions padded with kFarAway to a
+//               multiple of 8) and no self test outside the atom's own cell unless a short grid axis
+//               wraps a neighbour offset onto it.
+constexpr int kNlMetalLanes = 4;  // each takes 2 of every 8 candidates (8 = the slot padding)
+
+// Lists of metal atom i, G lanes per atom (l = lane within the group); called by whole groups.  Per
+// group step the lanes test candidates base + l and base + G + l of the cell (base a multiple of 8).
+template <bool M, int G, bool KEEP>
+__device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                         const CellBufs &bufs, int b, DevState *st, int *__restrict__ nlM,
+                                         int *__restrict__ nnM, int *__restrict__ fullNN,
+                                         int *__restrict__ fullNBL) {
+    static_assert(G == 4, "two candidates per lane cover a chunk of 8");
+    const int gt = blk * blockDim.x + threadIdx.x;
+    const int i = gt / G, l = gt % G;
+    const bool active = i < p.nMolMe;
+    const int c = active ? bufs.cellOf[b][i] : -1;
+    const int gshift = (threadIdx.x & 63) & ~(G - 1);
+    const unsigned gmask = (1u << G) - 1u, below = (1u << l) - 1u;
+    int nn = 0;
+    if (c >= 0) {
+        const int ccz = c / (p.cx * p.cy);
+        const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
+        const int ccx = c - (ccz * p.cy + ccy) * p.cx;
+        const float3 A = f3(pos[i]);
+        const float rrN = p.rrNebr;
+        const int *cnt = bufs.cnt[b];
+        const bool wrapSelf = p.cx < 3 || p.cy < 3 || p.cz < 3;
+        float shx, shy, shz;
+        int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
+        int ncur = cnt[C];
+        for (int o = 0; o < 27; ++o) {
+            float nshx = 0.f, nshy = 0.f, nshz = 0.f;
+            int Cn = C, nnext = 0;
+            if (o < 26) {  // prefetch the next cell's count
+                Cn = stencil_cell<float>(p, ccx, ccy, ccz, o + 1, nshx, nshy, nshz);
+                nnext = cnt[Cn];
+            }
+            const int n = min(ncur, p.cellCap);
+            const int *s = bufs.slots[b] + (size_t)C * p.cellCap;
+            const float4 *cp = bufs.cellPos + (size_t)C * p.cellCap;
+            const bool self = o == 0 || wrapSelf;
+            for (int base = 0; base < n; base += 2 * G) {
+                const int t0 = base + l, t1 = base + G + l;  // < n rounded up to 8
+                const int k0 = s[t0], k1 = s[t1];
+                const float4 q0 = cp[t0], q1 = cp[t1];
+                const float rr0 = shifted_rr(A, make_float3(q0.x, q0.y, q0.z), shx, shy, shz);
+                const float rr1 = shifted_rr(A, make_float3(q1.x, q1.y, q1.z), shx, shy, shz);
+                const bool h0 = rr0 < rrN && (!self || k0 != i);
+                const bool h1 = rr1 < rrN && (!self || k1 != i);
+                const unsigned g0 = (unsigned)(__ballot(h0) >> gshift) & gmask;
+                const unsigned g1 = (unsigned)(__ballot(h1) >> gshift) & gmask;
+                const int at0 = nn + __popc(g0 & below), at1 = nn + __popc(g0) + __popc(g1 & below);
+                if (h0 && at0 < p.nlCap) {
+                    nlM[(size_t)i * p.nlCap + at0] = k0;
+                    if (KEEP) fullNBL[(size_t)at0 * p.nMol + i] = k0;
+                }
+                if (h1 && at1 < p.nlCap) {
+                    nlM[(size_t)i * p.nlCap + at1] = k1;
+                    if (KEEP) fullNBL[(size_t)at1 * p.nMol + i] = k1;
+                }
+                nn += __popc(g0) + __popc(g1);
+            }
+            C = Cn;
+            ncur = nnext;
+            shx = nshx;
+            shy = nshy;
+            shz = nshz;
+        }
+    }
+    if (active && l == 0)
+        nl_finish(p, i, true, i - p.nMolMe, nn, 0, 0, nn > p.nlCap ? SGK_FLAG_NL_OVERFLOW : 0, nnM, nullptr, nullptr,
+                  fullNN, st);
+}
+
+// One chunk of 8 candidates of a carbon atom's walk (SELF: the cell may hold the atom itself).
+template <bool SELF, bool KEEP>
+__device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, const float3 &A, float shx, float shy,
+                                          float shz, const int *__restrict__ s, const float4 *__restrict__ cp,
+                                          int &nn, int &nljv, int &nclv, int &err, unsigned *__restrict__ lj,
+                                          unsigned *__restrict__ cl, int *__restrict__ fullNBL) {
+    const int4 k0 = *(const int4 *)s;
+    const int4 k1 = *(const int4 *)(s + 4);
+    float4 P[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) P[t] = cp[t];
+    const int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+    const float rrN = p.rrNebr, rrC = p.closeR2;
+    const int nMe = p.nMolMe;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const float rr = shifted_rr(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz);
+        bool hit = rr < rrN;
+        if (SELF) hit = hit && k[t] != i;
+        if (hit && (k[t] < nMe || rr < rrC)) {
+            if (nn < p.nlCap) {
+                const unsigned ent = ((unsigned)k[t] << 8) | (unsigned)nn;
+                if (k[t] < nMe) {
+                    if (nljv < p.ljCap)
+                        lj[(size_t)nljv * p.nC + ic] = ent;
+                    else
+                        err |= SGK_FLAG_LJ_OVERFLOW;
+                    ++nljv;
+                }
+                if (rr < rrC) {
+                    if (nclv < p.clCap)
+                        cl[(size_t)ic * p.clCap + nclv] = ent;
+                    else
+                        err |= SGK_FLAG_CLOSE_OVERFLOW;
+                    ++nclv;
+                }
+            }
+        }
+        if (KEEP && hit && nn < p.nlCap) fullNBL[(size_t)nn * p.nMol + i] = k[t];
+        nn += hit ? 1 : 0;
+    }
+}
+
+// Lists of carbon atom i in cell c (one lane).
+template <bool KEEP>
+__device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const float3 A, const int *__restrict__ cnt,
+                                         const int *__restrict__ slots, const float4 *__restrict__ cpos,
+                                         DevState *st, unsigned *__restrict__ lj, int *__restrict__ nlj,
+                                         unsigned *__restrict__ cl, int *__restrict__ ncl,
+                                         int *__restrict__ fullNN, int *__restrict__ fullNBL) {
+    const int ic = i - p.nMolMe;
+    const int ccz = c / (p.cx * p.cy);
+    const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
+    const int ccx = c - (ccz * p.cy + ccy) * p.cx;
+    const bool wrapSelf = p.cx < 3 || p.cy < 3 || p.cz < 3;
+    int nn = 0, nljv = 0, nclv = 0, err = 0;
+    float shx, shy, shz;
+    int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
+    int ncur = cnt[C];
+    for (int o = 0; o < 27; ++o) {
+        float nshx = 0.f, nshy = 0.f, nshz = 0.f;
+        int Cn = C, nnext = 0;
+        if (o < 26) {  // prefetch the next cell's count
+            Cn = stencil_cell<float>(p, ccx, ccy, ccz, o + 1, nshx, nshy, nshz);
+            nnext = cnt[Cn];
+        }
+        const int n = min(ncur, p.cellCap);
+        const int *s = slots + (size_t)C * p.cellCap;
+        const float4 *cp = cpos + (size_t)C * p.cellCap;
+        if (o == 0 || wrapSelf) {
+            for (int base = 0; base < n; base += 8)
+                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
+                                      fullNBL);
+        } else {
+            for (int base = 0; base < n; base += 8)
+                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
+                                       fullNBL);
+        }
+        C = Cn;
+        ncur = nnext;
+        shx = nshx;
+        shy = nshy;
+        shz = nshz;
+    }
+    if (nn > p.nlCap) err |= SGK_FLAG_NL_OVERFLOW;
+    nl_finish(p, i, false, ic, nn, nljv, nclv, err, nullptr, nlj, ncl, fullNN, st);
+}
+
+// Carbon part: lane t builds the lists of carbon order[t] (index order without an order).
+template <bool M, bool KEEP>
+__device__ __forceinline__ void nlc_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
+                                         const CellBufs &bufs, int b, DevState *st, const int *__restrict__ order,
+                                         unsigned *__restrict__ lj, int *__restrict__ nlj, unsigned *__restrict__ cl,
+                                         int *__restrict__ ncl, int *__restrict__ fullNN,
+                                         int *__restrict__ fullNBL) {
+    const int t = blk * blockDim.x + threadIdx.x;
+    if (t >= p.nC) return;
+    const int i = order ? order[t] : p.nMolMe + t;
+    const int c = bufs.cellOf[b][i];
+    if (c < 0) {  // not binned: empty lists
+        nl_finish(p, i, false, i - p.nMolMe, 0, 0, 0, 0, nullptr, nlj, ncl, fullNN, st);
+        return;
+    }
+    nlc_walk<KEEP>(p, i, c, f3(pos[i]), bufs.cnt[b], bufs.slots[b], bufs.cellPos, st, lj, nlj, cl, ncl, fullNN,
+                   fullNBL);
+}
+
+template <bool M, bool KEEP>
+__global__ __launch_bounds__(256) void k_nlc(DevParams p, int nbM, const typename Prec<M>::P4 *__restrict__ pos,
+                                             CellBufs bufs, DevState *st, int force, int fromNext,
+                                             const int *__restrict__ order, int *__restrict__ nlM,
+                                             int *__restrict__ nnM, unsigned *__restrict__ lj,
+                                             int *__restrict__ nlj, unsigned *__restrict__ cl,
+                                             int *__restrict__ ncl, int *__restrict__ fullNN,
+                                             int *__restrict__ fullNBL) {
+    if (!force && !st->nebrNow) return;
+    const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
+    if ((int)blockIdx.x < nbM)
+        nlm_body<M, kNlMetalLanes, KEEP>(p, blockIdx.x, pos, bufs, b, st, nlM, nnM, fullNN, fullNBL);
+    else
+        nlc_body<M, KEEP>(p, blockIdx.x - nbM, pos, bufs, b, st, order, lj, nlj, cl, ncl, fullNN, fullNBL);
+}
+
+// ---- carbon list-build order (nl_scheme 3) --------------------------------------------------------
+// A counting sort of the carbon atoms by the cell of their current position (the binning formula,
+// cells numbered column by column so that an atom drifting to the next z layer - a sheet lying on a
+// cell boundary splits across two - stays next to its column; bucket ncell for positions outside the
+// grid), then the metal atoms in index order, so that nlOrder[0, nC) are the carbons.  It is built
+// outside the step loop and ages as atoms move; it decides which lane builds which carbon's lists,
+// never the lists.
+constexpr int kOrdChunk = 4096;  // buckets per scan block, 16 per thread
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_ord_count(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                                   int *__restrict__ cnt, int *__restrict__ key,
+                                                   int *__restrict__ slot) {
+    const int n = p.nMolMe + blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMol) return;
+    int k;
+    {
+#pragma clang fp contract(off)
+        const float3 q = f3(pos[n]);
+        const float sx = q.x + 0.5f * p.Lx, sy = q.y + 0.5f * p.Ly, sz = q.z + 0.5f * p.Lz;
+        const int cx = (int)(sx * p.iwx), cy = (int)(sy * p.iwy), cz = (int)(sz * p.iwz);
+        const int c = (cy * p.cx + cx) * p.cz + cz;  // column-major: a column's z cells are adjacent
+        k = (c < 0 || c >= p.ncell) ? p.ncell : c;
+    }
+    // lanes with the same bucket share one atomicAdd, as in binning
+    const int lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(1), grp = 1ull << lane;
+    int leader = lane;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const int kl = __shfl(k, l, 64);
+        const unsigned long long m = __ballot(k == kl);
+        if (k == kl) {
+            grp = m;
+            leader = l;
+        }
+        todo &= ~m;
+    }
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[k], __popcll(grp));
+    base = __shfl(base, leader, 64);
+    key[n] = k;
+    slot[n] = base + __popcll(grp & below);
+}
+
+// Total of each block of kOrdChunk buckets [0, nb).
+__global__ __launch_bounds__(256) void k_ord_blocksum(int nb, const int *__restrict__ cnt, int *__restrict__ blockSum) {
+    __shared__ int sh[256 + 1];
+    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 16;
+    int sum = 0;
+    for (int j = 0; j < 16; ++j)
+        if (base + j < nb) sum += cnt[base + j];
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    block_scan4(sh, blockDim.x);
+    if (threadIdx.x == 0) blockSum[blockIdx.x] = sh[blockDim.x];
+}
+
+// Counts -> exclusive starts, in place: the block totals before this block, then the block's own scan.
+__global__ __launch_bounds__(256) void k_ord_scan(int nb, int *__restrict__ cnt, const int *__restrict__ blockSum) {
+    __shared__ int sh[256 + 1];
+    int pre = 0;
+    for (int j = threadIdx.x; j < (int)blockIdx.x; j += blockDim.x) pre += blockSum[j];
+    sh[threadIdx.x] = pre;
+    __syncthreads();
+    block_scan4(sh, blockDim.x);
+    pre = sh[blockDim.x];
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 16;
+    int v[16], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        v[j] = base + j < nb ? cnt[base + j] : 0;
+        sum += v[j];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    block_scan4(sh, blockDim.x);
+    int run = pre + sh[threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (base + j < nb) cnt[base + j] = run;
+        run += v[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ord_scatter(DevParams p, const int *__restrict__ start,
+                                                     const int *__restrict__ key, const int *__restrict__ slot,
+                                                     int *__restrict__ order) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nMol) return;
+    if (n < p.nMolMe)
+        order[p.nC + n] = n;
+    else
+        order[start[key[n]] + slot[n]] = n;
+}
+
 // Batched: the step path (lists built from the new cell buffer before the commit).
 template <bool M>
 __global__ __launch_bounds__(256) void kb_nl(BatchArgs a) {
@@ -735,6 +1051,24 @@ void launch_nl(sgk_ctx *c, bool force) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMol, 256);
     const int fromNext = force ? 0 : 1;  // step path: build from the new buffer before the commit
+    if (c->dp.nlScheme == 3) {
+        const int nbM = (int)blocks_for((long long)c->dp.nMolMe * kNlMetalLanes, 256);
+        const unsigned gc = (unsigned)nbM + blocks_for(c->dp.nC, 256);
+        const int *order = c->orderValid ? c->nlOrder : nullptr;
+#define SGK_NLC(MM, KK, P4)                                                                                      \
+    hipLaunchKernelGGL((k_nlc<MM, KK>), dim3(gc), dim3(256), 0, c->stream, c->dp, nbM, (const P4 *)c->pos,           \
+                       cell_bufs(c), c->state, (int)force, fromNext, order, c->nlM, c->nnM, c->lj, c->nlj, c->cl,  \
+                       c->ncl, c->fullNN, c->fullNBL)
+        const bool keep = c->dp.keepFull != 0;
+        if (c->mixed) {
+            if (keep) SGK_NLC(true, true, double4); else SGK_NLC(true, false, double4);
+        } else {
+            if (keep) SGK_NLC(false, true, float4); else SGK_NLC(false, false, float4);
+        }
+#undef SGK_NLC
+        profile_end(c, "nl");
+        return;
+    }
     const bool tiled = c->dp.nlScheme == 2;
 #define SGK_NL(K, MM, P4)                                                                                         \
     hipLaunchKernelGGL(K<MM>, dim3(g), dim3(256), 0, c->stream, c->dp, (const P4 *)c->pos, cell_bufs(c), c->state, \
@@ -746,6 +1080,28 @@ void launch_nl(sgk_ctx *c, bool force) {
     }
 #undef SGK_NL
     profile_end(c, "nl");
+}
+
+void build_nl_order(sgk_ctx *c) {
+    const DevParams &p = c->dp;
+    if (!c->nlOrder || p.nC == 0) return;
+    const int nb = p.ncell + 1;
+    const unsigned nblk = blocks_for(nb, kOrdChunk);
+    profile_begin(c);
+    (void)hipMemsetAsync(c->ordCnt, 0, sizeof(int) * (size_t)nb, c->stream);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_ord_count<true>, dim3(blocks_for(p.nC, 256)), dim3(256), 0, c->stream, p,
+                           (const double4 *)c->pos, c->ordCnt, c->ordKey, c->ordSlot);
+    else
+        hipLaunchKernelGGL(k_ord_count<false>, dim3(blocks_for(p.nC, 256)), dim3(256), 0, c->stream, p,
+                           (const float4 *)c->pos, c->ordCnt, c->ordKey, c->ordSlot);
+    hipLaunchKernelGGL(k_ord_blocksum, dim3(nblk), dim3(256), 0, c->stream, nb, c->ordCnt, c->ordBlock);
+    hipLaunchKernelGGL(k_ord_scan, dim3(nblk), dim3(256), 0, c->stream, nb, c->ordCnt, c->ordBlock);
+    hipLaunchKernelGGL(k_ord_scatter, dim3(blocks_for(p.nMol, 256)), dim3(256), 0, c->stream, p, c->ordCnt, c->ordKey,
+                       c->ordSlot, c->nlOrder);
+    profile_end(c, "nlorder");
+    c->orderValid = true;
+    c->orderAge = 0;
 }
 
 void launch_batch_sortcells(sgk_batch *b) {

@@ -177,6 +177,10 @@ class Engine:
     def build_nebr(self):
         self._check(self.lib.sgk_build_nebr(self.h), "sgk_build_nebr")
 
+    def build_nl_order(self):
+        """nl_scheme 3: rebuild the carbon list-build order from the current positions now."""
+        self._check(self.lib.sgk_build_nl_order(self.h), "sgk_build_nl_order")
+
     def get_nebr(self):
         cap = self.cfg.nl_capacity or self.params.iNebrMax
         nn = np.zeros(self.n, np.int32)

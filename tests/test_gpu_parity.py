@@ -42,10 +42,12 @@ def force_error(acc, ref):
     return np.abs(acc - ref).max() / rms
 
 
-def check_lists(p, prec, scheme, r):
+def check_lists(p, prec, scheme, r, ordered=False):
     o = oracle.Oracle(p, "f32")
     with sg.Engine(p, precision=prec, keep_full_nl=True, nl_scheme=scheme) as eng:
         eng.upload(r)
+        if ordered:
+            eng.build_nl_order()
         eng.bin()
         cell_of, cnt, slots = eng.get_cells()
         err, ocell, ocnt, oslots = o.bin(r)
@@ -65,17 +67,26 @@ def check_lists(p, prec, scheme, r):
 
 @pytest.mark.parametrize("name", ["small", "icosa", "medium"])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
-@pytest.mark.parametrize("scheme", [2, 1, 0])
+@pytest.mark.parametrize("scheme", [3, 2, 1, 0])
 def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
-    """Every list scheme (LDS-staged per workgroup, one wave per cell, one thread per atom) gives the
-    reference list."""
+    """Every list scheme (owned cells with lanes per metal atom, LDS-staged per workgroup, one wave per
+    cell, one thread per atom) gives the reference list."""
     s = system(name)
     print(f"[parity] cells/NL {name} {prec} scheme {scheme}", flush=True)
     check_lists(s.params, prec, scheme, s.r)
 
 
+@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_cell_ordered_neighbour_list_bit_exact(gpu, name, prec):
+    """nl_scheme 3 with its carbon order built from the same positions."""
+    s = system(name)
+    print(f"[parity] cells/NL {name} {prec} scheme 3, cell order", flush=True)
+    check_lists(s.params, prec, 3, s.r, ordered=True)
+
+
 @pytest.mark.parametrize("name", ["icosa", "medium"])
-@pytest.mark.parametrize("scheme", [2, 0])
+@pytest.mark.parametrize("scheme", [3, 2, 0])
 def test_neighbour_list_bit_exact_thermalised(gpu, name, scheme):
     """On a thermalised state the sheet straddles a cell boundary in z and the cluster has moved:
     more occupied cells per workgroup box, and the metal workgroups exceed the LDS budget, so the
@@ -83,7 +94,62 @@ def test_neighbour_list_bit_exact_thermalised(gpu, name, scheme):
     s = system(name)
     r, _, _ = thermalised(name)
     print(f"[parity] NL thermalised {name} scheme {scheme}", flush=True)
-    check_lists(s.params, "mixed", scheme, r)
+    check_lists(s.params, "mixed", scheme, r, ordered=scheme == 3)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("scheme", [3, 0])
+def test_neighbour_list_bit_exact_wrapped_grid(gpu, prec, scheme):
+    """32 Ni atoms in a 1 x 1 x 1 cell grid: all 27 offsets wrap onto the atom's own cell with a
+    shift, so every offset holds the atom itself and the other atoms' periodic images."""
+    s = sg.make_system(regime=0, material=3, ux=2, uy=2, uz=2, temperature=600.0, velocities="maxwell", seed=2)
+    p = s.params
+    assert (p.cells.x, p.cells.y, p.cells.z) == (1, 1, 1)
+    print(f"[parity] NL wrapped grid {prec} scheme {scheme}", flush=True)
+    check_lists(p, prec, scheme, s.r)
+
+
+@pytest.mark.parametrize("name", ["icosa", "medium"])
+def test_cell_ordered_lists_after_run(gpu, name):
+    """nl_scheme 3 after a run: the carbon lists are built in the cell order made at the start of the
+    run (from positions that have since moved), and still equal the reference list."""
+    s = system(name)
+    p = s.params
+    print(f"[parity] cell-ordered NL after 40 steps, {name}", flush=True)
+    with sg.Engine(p, precision="mixed", keep_full_nl=True, rebuild_every=10, nl_scheme=3) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.set_run_state(0, 1, 0.0)
+        eng.run(40, records=False)
+        r, _, _ = eng.download()
+        eng.bin()
+        eng.build_nebr()
+        nn, nbl = eng.get_nebr()
+        o = oracle.Oracle(p, "f32")
+        err, ocell, ocnt, oslots = o.bin(r)
+        assert err == 0
+        err, onn, onbl = o.nebr(r, ocnt, oslots)
+        assert err == 0
+        assert np.array_equal(nn, onn)
+        assert np.array_equal(nbl, onbl)
+        check_carbon_lists(eng, p, np.asarray(r, np.float32), onn, onbl)
+
+
+@pytest.mark.parametrize("name", ["icosa", "medium"])
+def test_list_schemes_same_trajectory(gpu, name):
+    """The owned-cell scheme inside the device-resident step loop (lists built from the new cell
+    buffer before the commit, skin-rule and fixed-cadence rebuilds) gives the per-atom scheme's
+    trajectory bit for bit."""
+    s = system(name)
+    print(f"[parity] list schemes, 300 steps, {name}", flush=True)
+    out = []
+    for scheme in (0, 3):
+        with sg.Engine(s.params, precision="mixed", rebuild_every=10, nl_scheme=scheme) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            eng.run(300, records=False)
+            out.append(eng.download())
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
 
 
 def check_carbon_lists(eng, p, r, nn, nbl):
