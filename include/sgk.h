@@ -121,8 +121,8 @@ int sgk_bin(sgk_ctx *ctx);
 int sgk_get_cells(sgk_ctx *ctx, int *cellOf, int *cellCount, int *cellSlots);
 /* Neighbour list (BuildNebrListK, SurfaceGrowth.cu:612-711) from the current cells. */
 int sgk_build_nebr(sgk_ctx *ctx);
-/* nl_scheme 3: rebuild now the order in which the list build visits carbon atoms (by the cell of
- * their current position).  sgk_run rebuilds it when it is missing or 100 steps old; the order
+/* nl_scheme 3: rebuild now the order in which the list build visits atoms (by the cell of their
+ * current position).  sgk_run rebuilds it when it is missing or 100 steps old; the order
  * changes which lane builds which list, never the lists.  No counterpart in the reference (its list
  * build has one thread per atom in index order).  A no-op for the other schemes. */
 int sgk_build_nl_order(sgk_ctx *ctx);

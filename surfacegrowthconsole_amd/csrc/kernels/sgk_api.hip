@@ -330,12 +330,13 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     SGK_TRY(dalloc(c, &c->ncl, nC, true));
     SGK_TRY(dalloc(c, &c->bondC, nC, true));
     SGK_TRY(dalloc(c, &c->bondA, nC, true));
-    if (p.nlScheme == 3 && nC > 0) {
+    if (p.nlScheme == 3 && nMol > 0) {
+        const size_t nb = 2 * (size_t)p.ncell + 2;  // carbon and metal buckets of the list-build order
         SGK_TRY(dalloc(c, &c->nlOrder, nMol, false));
-        SGK_TRY(dalloc(c, &c->ordCnt, (size_t)p.ncell + 1, false));
+        SGK_TRY(dalloc(c, &c->ordCnt, nb, false));
         SGK_TRY(dalloc(c, &c->ordKey, nMol, false));
         SGK_TRY(dalloc(c, &c->ordSlot, nMol, false));
-        SGK_TRY(dalloc(c, &c->ordBlock, ((size_t)p.ncell + 1 + 4095) / 4096, false));
+        SGK_TRY(dalloc(c, &c->ordBlock, (nb + 4095) / 4096, false));
     }
     if (p.keepFull) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
@@ -818,7 +819,7 @@ int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, do
     if (adh && (!c->adj || c->adj->every < nsteps)) return fail(c, SGK_E_STATE, "no adhesion-energy buffer for the run");
     const std::string flags = sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, adh != nullptr);
     SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    // the carbon list-build order of nl_scheme 3 follows the cells of the current positions; it decides
+    // the list-build order of nl_scheme 3 follows the cells of the current positions; it decides
     // only which lane builds which atom's lists, so it is refreshed now and then, outside the graph
     if (c->nlOrder && (!c->orderValid || c->orderAge >= sgk::kNlOrderSteps)) sgk::build_nl_order(c);
     if (c->cfg.use_graphs && !c->profiling) {

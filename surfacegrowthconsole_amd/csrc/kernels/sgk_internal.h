@@ -164,7 +164,7 @@ void launch_batch_rdf(sgk_batch *b);
 void launch_sortcells(sgk_ctx *c, bool force);
 void launch_nl(sgk_ctx *c, bool force);
 void build_nl_order(sgk_ctx *c);
-constexpr int kNlOrderSteps = 100;  // age (steps) at which sgk_run rebuilds the nl_scheme 3 carbon order
+constexpr int kNlOrderSteps = 100;  // age (steps) at which sgk_run rebuilds the nl_scheme 3 list-build order
 void launch_commit_rebuild(sgk_ctx *c);
 void launch_rdf(sgk_ctx *c);
 // sgk_force.hip

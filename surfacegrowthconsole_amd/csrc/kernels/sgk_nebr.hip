@@ -621,12 +621,13 @@ __global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>
 // bit.  The per-atom walk above is bound at once by the L1 (a wave's 64 atoms lie in ~20 different
 // cells, so each gather touches ~20 cache lines), by VALU issue and by the scalar unit, and its metal
 // atoms - ~480 candidates each, one lane per atom - finish last.  Here:
-//   metal part  (blocks [0, nbM)): kNlMetalLanes lanes per metal atom, atoms in index order.  The
-//               lanes of an atom take two candidates each of every chunk of 8 in the walk order; a
-//               ballot over the group gives each hit its list position.
-//   carbon part (blocks [nbM, nbM + nbC)): one lane per carbon, carbons in the order of their cells
-//               (nlOrder, built by build_nl_order), so the lanes of a wave hold a few adjacent cells
-//               that share their neighbour cells and their candidate loads hit the same cache lines.
+//   metal part  (blocks [0, nbM)): kNlMetalLanes lanes per metal atom.  The lanes of an atom take
+//               two candidates each of every chunk of 8 in the walk order; a ballot over the group
+//               gives each hit its list position.
+//   carbon part (blocks [nbM, nbM + nbC)): one lane per carbon.
+//   Both visit their atoms in the order of their cells (nlOrder, built by build_nl_order; index
+//   order without one), so the lanes of a wave hold a few adjacent cells that share their neighbour
+//   cells and their candidate loads hit the same cache lines.
 //               The walk has no per-candidate bound test (positions padded with kFarAway to a
 //               multiple of 8) and no self test outside the atom's own cell unless a short grid axis
 //               wraps a neighbour offset onto it.
@@ -636,13 +637,14 @@ constexpr int kNlMetalLanes = 4;  // each takes 2 of every 8 candidates (8 = the
 // group step the lanes test candidates base + l and base + G + l of the cell (base a multiple of 8).
 template <bool M, int G, bool KEEP>
 __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
-                                         const CellBufs &bufs, int b, DevState *st, int *__restrict__ nlM,
-                                         int *__restrict__ nnM, int *__restrict__ fullNN,
+                                         const CellBufs &bufs, int b, DevState *st, const int *__restrict__ order,
+                                         int *__restrict__ nlM, int *__restrict__ nnM, int *__restrict__ fullNN,
                                          int *__restrict__ fullNBL) {
     static_assert(G == 4, "two candidates per lane cover a chunk of 8");
     const int gt = blk * blockDim.x + threadIdx.x;
-    const int i = gt / G, l = gt % G;
-    const bool active = i < p.nMolMe;
+    const int g = gt / G, l = gt % G;
+    const bool active = g < p.nMolMe;
+    const int i = active && order ? order[p.nC + g] : g;  // metal atoms by cell (index order without one)
     const int c = active ? bufs.cellOf[b][i] : -1;
     const int gshift = (threadIdx.x & 63) & ~(G - 1);
     const unsigned gmask = (1u << G) - 1u, below = (1u << l) - 1u;
@@ -820,25 +822,25 @@ __global__ __launch_bounds__(256) void k_nlc(DevParams p, int nbM, const typenam
     if (!force && !st->nebrNow) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
     if ((int)blockIdx.x < nbM)
-        nlm_body<M, kNlMetalLanes, KEEP>(p, blockIdx.x, pos, bufs, b, st, nlM, nnM, fullNN, fullNBL);
+        nlm_body<M, kNlMetalLanes, KEEP>(p, blockIdx.x, pos, bufs, b, st, order, nlM, nnM, fullNN, fullNBL);
     else
         nlc_body<M, KEEP>(p, blockIdx.x - nbM, pos, bufs, b, st, order, lj, nlj, cl, ncl, fullNN, fullNBL);
 }
 
-// ---- carbon list-build order (nl_scheme 3) --------------------------------------------------------
-// A counting sort of the carbon atoms by the cell of their current position (the binning formula,
-// cells numbered column by column so that an atom drifting to the next z layer - a sheet lying on a
-// cell boundary splits across two - stays next to its column; bucket ncell for positions outside the
-// grid), then the metal atoms in index order, so that nlOrder[0, nC) are the carbons.  It is built
-// outside the step loop and ages as atoms move; it decides which lane builds which carbon's lists,
-// never the lists.
+// ---- list-build order (nl_scheme 3) ----------------------------------------------------------------
+// A counting sort of the atoms by the cell of their current position (the binning formula, cells
+// numbered column by column so that an atom drifting to the next z layer - a sheet lying on a cell
+// boundary splits across two - stays next to its column; one bucket for positions outside the grid):
+// carbons in buckets [0, ncell], metal atoms in [ncell + 1, 2 ncell + 1], so that nlOrder[0, nC) are
+// the carbons and nlOrder[nC, nMol) the metal atoms.  It is built outside the step loop and ages as
+// atoms move; it decides which lanes build which atom's lists, never the lists.
 constexpr int kOrdChunk = 4096;  // buckets per scan block, 16 per thread
 
 template <bool M>
 __global__ __launch_bounds__(256) void k_ord_count(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
                                                    int *__restrict__ cnt, int *__restrict__ key,
                                                    int *__restrict__ slot) {
-    const int n = p.nMolMe + blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
     int k;
     {
@@ -848,6 +850,7 @@ __global__ __launch_bounds__(256) void k_ord_count(DevParams p, const typename P
         const int cx = (int)(sx * p.iwx), cy = (int)(sy * p.iwy), cz = (int)(sz * p.iwz);
         const int c = (cy * p.cx + cx) * p.cz + cz;  // column-major: a column's z cells are adjacent
         k = (c < 0 || c >= p.ncell) ? p.ncell : c;
+        if (n < p.nMolMe) k += p.ncell + 1;  // metal buckets after the carbon ones
     }
     // lanes with the same bucket share one atomicAdd, as in binning
     const int lane = __lane_id();
@@ -917,10 +920,7 @@ __global__ __launch_bounds__(256) void k_ord_scatter(DevParams p, const int *__r
                                                      int *__restrict__ order) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
-    if (n < p.nMolMe)
-        order[p.nC + n] = n;
-    else
-        order[start[key[n]] + slot[n]] = n;
+    order[start[key[n]] + slot[n]] = n;
 }
 
 // Batched: the step path (lists built from the new cell buffer before the commit).
@@ -1084,16 +1084,16 @@ void launch_nl(sgk_ctx *c, bool force) {
 
 void build_nl_order(sgk_ctx *c) {
     const DevParams &p = c->dp;
-    if (!c->nlOrder || p.nC == 0) return;
-    const int nb = p.ncell + 1;
+    if (!c->nlOrder || p.nMol == 0) return;
+    const int nb = 2 * p.ncell + 2;
     const unsigned nblk = blocks_for(nb, kOrdChunk);
     profile_begin(c);
     (void)hipMemsetAsync(c->ordCnt, 0, sizeof(int) * (size_t)nb, c->stream);
     if (c->mixed)
-        hipLaunchKernelGGL(k_ord_count<true>, dim3(blocks_for(p.nC, 256)), dim3(256), 0, c->stream, p,
+        hipLaunchKernelGGL(k_ord_count<true>, dim3(blocks_for(p.nMol, 256)), dim3(256), 0, c->stream, p,
                            (const double4 *)c->pos, c->ordCnt, c->ordKey, c->ordSlot);
     else
-        hipLaunchKernelGGL(k_ord_count<false>, dim3(blocks_for(p.nC, 256)), dim3(256), 0, c->stream, p,
+        hipLaunchKernelGGL(k_ord_count<false>, dim3(blocks_for(p.nMol, 256)), dim3(256), 0, c->stream, p,
                            (const float4 *)c->pos, c->ordCnt, c->ordKey, c->ordSlot);
     hipLaunchKernelGGL(k_ord_blocksum, dim3(nblk), dim3(256), 0, c->stream, nb, c->ordCnt, c->ordBlock);
     hipLaunchKernelGGL(k_ord_scan, dim3(nblk), dim3(256), 0, c->stream, nb, c->ordCnt, c->ordBlock);

@@ -178,7 +178,7 @@ class Engine:
         self._check(self.lib.sgk_build_nebr(self.h), "sgk_build_nebr")
 
     def build_nl_order(self):
-        """nl_scheme 3: rebuild the carbon list-build order from the current positions now."""
+        """nl_scheme 3: rebuild the list-build order from the current positions now."""
         self._check(self.lib.sgk_build_nl_order(self.h), "sgk_build_nl_order")
 
     def get_nebr(self):
