@@ -785,14 +785,20 @@ std::string sgk::step_flag_string(sgk_ctx *c, int step0, int nsteps, int rdfEver
 
 static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
     sgk::launch_reset_records(c);
-    for (char ch : flags) {
-        const int b = ch - 'a';
+    // The force pass of a step also applies the next step's first half-kick when nothing touches the
+    // velocities or accelerations in between (no thermostat, shear or deposition) and the step is
+    // not the chunk's last (the host then reads full-step velocities): the next k_lf1 only drifts.
+    bool kicked = false;
+    for (size_t k = 0; k < flags.size(); ++k) {
+        const int b = flags[k] - 'a';
+        const bool prekick = c->dp.iRegime != 1 && !(b & 3) && k + 1 < flags.size();
         sgk::launch_insert(c);  // regime 1 only
-        sgk::launch_lf1(c, true);
+        sgk::launch_lf1(c, true, kicked);
         sgk::launch_sortcells(c, false);
         sgk::launch_nl(c, false);
         sgk::launch_rho(c);
-        sgk::launch_force(c, true, true);
+        sgk::launch_force(c, true, true, prekick);
+        kicked = prekick;
         if (b & 1) sgk::launch_thermostat(c, 0, 0.0, 0);
         if (b & 2) sgk::launch_shear(c);
         if (b & 4) sgk::launch_rdf(c);

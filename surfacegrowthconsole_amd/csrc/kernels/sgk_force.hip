@@ -249,9 +249,11 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
     partial_identity(q);
     if (active && l == 0) {
         const R ax = (R)(fx / (T)p.massMe), ay = (R)(fy / (T)p.massMe), az = (R)(fz / (T)p.massMe);
-        acc[i] = ax;
-        acc[p.nMol + i] = ay;
-        acc[2 * p.nMol + i] = az;
+        if (integrate != 2) {  // after the next step's half-kick here, nothing reads it before it is rewritten
+            acc[i] = ax;
+            acc[p.nMol + i] = ay;
+            acc[2 * p.nMol + i] = az;
+        }
         const T e = T(0.5f) * u + (T)embF[i];
         pos[i].w = (R)e;
         fric[i] = (float)frx;
@@ -262,9 +264,10 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
             vx = vx + hdt * ax;
             vy = vy + hdt * ay;
             vz = vz + hdt * az;
-            vel[i] = vx;
-            vel[p.nMol + i] = vy;
-            vel[2 * p.nMol + i] = vz;
+            // integrate 2: also the next step's first half-kick, the same operations k_lf1 would do
+            vel[i] = integrate == 2 ? vx + hdt * ax : vx;
+            vel[p.nMol + i] = integrate == 2 ? vy + hdt * ay : vy;
+            vel[2 * p.nMol + i] = integrate == 2 ? vz + hdt * az : vz;
         }
         const A m = (A)p.massMe;
         q.u = (A)e;
@@ -468,9 +471,11 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         }
     }
     const R ax = (R)gx, ay = (R)gy, az = (R)gz;  // carbon mass is the unit
-    acc[i] = ax;
-    acc[p.nMol + i] = ay;
-    acc[2 * p.nMol + i] = az;
+    if (integrate != 2) {  // after the next step's half-kick here, nothing reads it before it is rewritten
+        acc[i] = ax;
+        acc[p.nMol + i] = ay;
+        acc[2 * p.nMol + i] = az;
+    }
     const float e = (float)(T(0.5f) * gu);
     pos[i].w = (R)e;
     R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
@@ -481,9 +486,9 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         vx = vx + hdt * ax;
         vy = vy + hdt * ay;
         vz = vz + hdt * az;
-        vel[i] = vx;
-        vel[p.nMol + i] = vy;
-        vel[2 * p.nMol + i] = vz;
+        vel[i] = integrate == 2 ? vx + hdt * ax : vx;  // 2: with the next step's first half-kick
+        vel[p.nMol + i] = integrate == 2 ? vy + hdt * ay : vy;
+        vel[2 * p.nMol + i] = integrate == 2 ? vz + hdt * az : vz;
     }
     q.u = (A)e;
     q.vs[0] = (A)vx;
@@ -570,7 +575,7 @@ static void rho_dispatch(sgk_ctx *c) {
 }
 
 template <bool M, class T>
-static void force_dispatch(sgk_ctx *c, bool integrate) {
+static void force_dispatch(sgk_ctx *c, int integrate) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
@@ -582,7 +587,7 @@ static void force_dispatch(sgk_ctx *c, bool integrate) {
 #define SGK_F(GG)                                                                                                \
     hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, (R *)c->acc, \
                        c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->bondC, c->bondA, part,     \
-                       (int)integrate)
+                       integrate)
     switch (G) {
         case 1: SGK_F(1); break;
         case 2: SGK_F(2); break;
@@ -638,15 +643,16 @@ static void batch_launch(sgk_batch *b, bool force) {
 void launch_batch_rho(sgk_batch *b) { batch_launch(b, false); }
 void launch_batch_force(sgk_batch *b) { batch_launch(b, true); }
 
-void launch_force(sgk_ctx *c, bool integrate, bool finalize) {
+void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick) {
     if (c->nbM + c->nbC == 0) return;
     profile_begin(c);
+    const int mode = integrate ? (prekick ? 2 : 1) : 0;
     if (!c->mixed)
-        force_dispatch<false, float>(c, integrate);
+        force_dispatch<false, float>(c, mode);
     else if (c->cfg.pair_fp64)
-        force_dispatch<true, double>(c, integrate);
+        force_dispatch<true, double>(c, mode);
     else
-        force_dispatch<true, float>(c, integrate);
+        force_dispatch<true, float>(c, mode);
     profile_end(c, "force");
     if (finalize) launch_finalize(c, 0);
 }

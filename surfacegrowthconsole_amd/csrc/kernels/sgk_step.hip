@@ -16,11 +16,13 @@ namespace sgk {
 // LeapfrogStepK part 1 (SurfaceGrowth.cu:397-434) fused with ApplyBoundaryCondK (436-453) and,
 // when the device flag asks for a rebuild, with the cell binning of the new position
 // (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
+// kicked: the previous force pass already applied this step's half-kick (integrate 2), so only the
+// drift is left.
 template <bool M>
 __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos,
                                          typename Prec<M>::R *__restrict__ vel,
                                          const typename Prec<M>::R *__restrict__ acc, const CellBufs &bufs,
-                                         DevState *st, int doBin) {
+                                         DevState *st, int doBin, int kicked) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
     const int n = blk * blockDim.x + threadIdx.x;
@@ -32,12 +34,14 @@ __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename P
     if (move) {
         const R hdt = (R)(0.5f * p.deltaT), dt = (R)p.deltaT;
         R vx = vel[n], vy = vel[p.nMol + n], vz = vel[2 * p.nMol + n];
-        vx = vx + hdt * acc[n];
-        vy = vy + hdt * acc[p.nMol + n];
-        vz = vz + hdt * acc[2 * p.nMol + n];
-        vel[n] = vx;
-        vel[p.nMol + n] = vy;
-        vel[2 * p.nMol + n] = vz;
+        if (!kicked) {
+            vx = vx + hdt * acc[n];
+            vy = vy + hdt * acc[p.nMol + n];
+            vz = vz + hdt * acc[2 * p.nMol + n];
+            vel[n] = vx;
+            vel[p.nMol + n] = vy;
+            vel[2 * p.nMol + n] = vz;
+        }
         r.x = r.x + dt * vx;
         r.y = r.y + dt * vy;
         r.z = r.z + dt * vz;
@@ -57,8 +61,8 @@ template <bool M>
 __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *__restrict__ pos,
                                              typename Prec<M>::R *__restrict__ vel,
                                              const typename Prec<M>::R *__restrict__ acc,
-                                             CellBufs bufs, DevState *st, int doBin) {
-    lf1_body<M>(p, blockIdx.x, pos, vel, acc, bufs, st, doBin);
+                                             CellBufs bufs, DevState *st, int doBin, int kicked) {
+    lf1_body<M>(p, blockIdx.x, pos, vel, acc, bufs, st, doBin, kicked);
 }
 
 // Batched: one workgroup of one replica per table entry.
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(256) void kb_lf1(BatchArgs a) {
     const int2 e = a.atomTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
     lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
-                rep_cells(a, B), a.state + e.x, 1);
+                rep_cells(a, B), a.state + e.x, 1, 0);
 }
 
 // InsertAtomsK (SurfaceGrowth.cu:488-514) with the schedule of .cu:2148-2168, for the step about to
@@ -289,15 +293,15 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 size_t partial_bytes_for(bool mixed) { return mixed ? sizeof(Partial<double>) : sizeof(Partial<float>); }
 size_t partial_bytes(const sgk_ctx *c) { return partial_bytes_for(c->mixed); }
 
-void launch_lf1(sgk_ctx *c, bool bin) {
+void launch_lf1(sgk_ctx *c, bool bin, bool kicked) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMol, 256);
     if (c->mixed)
         hipLaunchKernelGGL(k_lf1<true>, dim3(g), dim3(256), 0, c->stream, c->dp, (double4 *)c->pos, (double *)c->vel,
-                           (const double *)c->acc, cell_bufs(c), c->state, (int)bin);
+                           (const double *)c->acc, cell_bufs(c), c->state, (int)bin, (int)kicked);
     else
         hipLaunchKernelGGL(k_lf1<false>, dim3(g), dim3(256), 0, c->stream, c->dp, (float4 *)c->pos, (float *)c->vel,
-                           (const float *)c->acc, cell_bufs(c), c->state, (int)bin);
+                           (const float *)c->acc, cell_bufs(c), c->state, (int)bin, (int)kicked);
     profile_end(c, "lf1");
 }
 
