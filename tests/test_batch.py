@@ -36,7 +36,7 @@ def test_batch_rejects_replicas_with_different_potentials():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
-@pytest.mark.parametrize("nl_scheme", [2, 0])
+@pytest.mark.parametrize("nl_scheme", [3, 2, 0])
 def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
     systems = replicas()
     steps = 40

@@ -135,15 +135,16 @@ def test_cell_ordered_lists_after_run(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["icosa", "medium"])
-def test_list_schemes_same_trajectory(gpu, name):
-    """The owned-cell scheme inside the device-resident step loop (lists built from the new cell
-    buffer before the commit, skin-rule and fixed-cadence rebuilds) gives the per-atom scheme's
-    trajectory bit for bit."""
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_list_schemes_same_trajectory(gpu, name, prec):
+    """The cell-ordered scheme inside the device-resident step loop (lists built from the new cell
+    buffer before the commit, skin-rule and fixed-cadence rebuilds, the order built at the start of
+    the run) gives the per-atom scheme's trajectory bit for bit."""
     s = system(name)
-    print(f"[parity] list schemes, 300 steps, {name}", flush=True)
+    print(f"[parity] list schemes, 300 steps, {name} {prec}", flush=True)
     out = []
     for scheme in (0, 3):
-        with sg.Engine(s.params, precision="mixed", rebuild_every=10, nl_scheme=scheme) as eng:
+        with sg.Engine(s.params, precision=prec, rebuild_every=10, nl_scheme=scheme) as eng:
             eng.upload(s.r, s.v, s.a)
             eng.set_run_state(0, 1, 0.0)
             eng.run(300, records=False)
