@@ -94,14 +94,19 @@ std::string batch_flags(const sgk_batch *b, int step0, int nsteps, int rdfEvery)
 
 void enqueue_batch(sgk_batch *b, const std::string &flags) {
     sgk::launch_batch_reset_records(b);
-    for (char ch : flags) {
-        const int k = ch - 'a';
-        sgk::launch_batch_lf1(b);
+    // as enqueue_steps: the force pass applies the next step's first half-kick when no thermostat or
+    // shear comes in between and the step is not the chunk's last
+    bool kicked = false;
+    for (size_t s = 0; s < flags.size(); ++s) {
+        const int k = flags[s] - 'a';
+        const bool prekick = !(k & 3) && s + 1 < flags.size();
+        sgk::launch_batch_lf1(b, kicked);
         sgk::launch_batch_sortcells(b);
         sgk::launch_batch_nl(b);
         sgk::launch_batch_rho(b);
-        sgk::launch_batch_force(b);
+        sgk::launch_batch_force(b, prekick);
         sgk::launch_batch_finalize(b);
+        kicked = prekick;
         if (k & 1) sgk::launch_batch_thermostat(b);
         if (k & 2) sgk::launch_batch_shear(b);
         if (k & 4) sgk::launch_batch_rdf(b);

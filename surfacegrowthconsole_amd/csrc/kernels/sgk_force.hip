@@ -540,7 +540,7 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
 
 // Batched: each replica's workgroups write into its own range of partials (kb_finalize reduces).
 template <bool M, class T, int G>
-__global__ __launch_bounds__(256) void kb_force(BatchArgs a) {
+__global__ __launch_bounds__(256) void kb_force(BatchArgs a, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
     const int2 e = a.forceTab[blockIdx.x];
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) void kb_force(BatchArgs a) {
                          (R *)a.vel + 3 * B.atomOff, (R *)a.acc + 3 * B.atomOff, a.nlM + B.metalOff * p.nlCap,
                          a.nnM + B.metalOff, a.embF + B.metalOff, a.embDF + B.metalOff, a.fric + B.metalOff,
                          a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.bondC + B.carbonOff,
-                         a.bondA + B.carbonOff, (Partial<A> *)a.partials + B.partOff, 1);
+                         a.bondA + B.carbonOff, (Partial<A> *)a.partials + B.partOff, integrate);
 }
 
 // ---- launchers ---------------------------------------------------------------------------------
@@ -611,12 +611,12 @@ void launch_rho(sgk_ctx *c) {
 }
 
 template <bool M, class T>
-static void batch_dispatch(sgk_batch *b, bool force) {
+static void batch_dispatch(sgk_batch *b, bool force, int integrate) {
     const dim3 g(b->nForceBlocks), t(256);
 #define SGK_B(GG)                                                                            \
     do {                                                                                     \
         if (force)                                                                           \
-            hipLaunchKernelGGL((kb_force<M, T, GG>), g, t, 0, b->stream, b->args());         \
+            hipLaunchKernelGGL((kb_force<M, T, GG>), g, t, 0, b->stream, b->args(), integrate); \
         else                                                                                 \
             hipLaunchKernelGGL((kb_rho<M, T, GG>), g, t, 0, b->stream, b->args());           \
     } while (0)
@@ -630,18 +630,18 @@ static void batch_dispatch(sgk_batch *b, bool force) {
 #undef SGK_B
 }
 
-static void batch_launch(sgk_batch *b, bool force) {
+static void batch_launch(sgk_batch *b, bool force, int integrate) {
     if (b->nForceBlocks == 0) return;
     if (!b->mixed)
-        batch_dispatch<false, float>(b, force);
+        batch_dispatch<false, float>(b, force, integrate);
     else if (b->cfg.pair_fp64)
-        batch_dispatch<true, double>(b, force);
+        batch_dispatch<true, double>(b, force, integrate);
     else
-        batch_dispatch<true, float>(b, force);
+        batch_dispatch<true, float>(b, force, integrate);
 }
 
-void launch_batch_rho(sgk_batch *b) { batch_launch(b, false); }
-void launch_batch_force(sgk_batch *b) { batch_launch(b, true); }
+void launch_batch_rho(sgk_batch *b) { batch_launch(b, false, 0); }
+void launch_batch_force(sgk_batch *b, bool prekick) { batch_launch(b, true, prekick ? 2 : 1); }
 
 void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick) {
     if (c->nbM + c->nbC == 0) return;

@@ -170,14 +170,14 @@ void launch_rdf(sgk_ctx *c);
 // sgk_force.hip
 void launch_rho(sgk_ctx *c);
 void launch_batch_rho(sgk_batch *b);
-void launch_batch_force(sgk_batch *b);
+void launch_batch_force(sgk_batch *b, bool prekick = false);
 // prekick: the force pass also applies the next step's first half-kick (the caller guarantees that
 // nothing touches velocities or accelerations before that step's launch_lf1(kicked = true)).
 void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick = false);
 // sgk_step.hip
 void launch_lf1(sgk_ctx *c, bool bin, bool kicked = false);
 void launch_insert(sgk_ctx *c);
-void launch_batch_lf1(sgk_batch *b);
+void launch_batch_lf1(sgk_batch *b, bool kicked = false);
 void launch_batch_finalize(sgk_batch *b);
 void launch_batch_thermostat(sgk_batch *b);
 void launch_batch_shear(sgk_batch *b);

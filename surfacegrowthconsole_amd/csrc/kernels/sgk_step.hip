@@ -67,13 +67,13 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
 
 // Batched: one workgroup of one replica per table entry.
 template <bool M>
-__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a) {
+__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     const int2 e = a.atomTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
     lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
-                rep_cells(a, B), a.state + e.x, 1, 0);
+                rep_cells(a, B), a.state + e.x, 1, kicked);
 }
 
 // InsertAtomsK (SurfaceGrowth.cu:488-514) with the schedule of .cu:2148-2168, for the step about to
@@ -397,11 +397,11 @@ void launch_shear(sgk_ctx *c) {
 void launch_reset_records(sgk_ctx *c) { hipLaunchKernelGGL(k_reset_records, dim3(1), dim3(1), 0, c->stream, c->state); }
 
 // ---- batched launchers -----------------------------------------------------------------------
-void launch_batch_lf1(sgk_batch *b) {
+void launch_batch_lf1(sgk_batch *b, bool kicked) {
     if (b->mixed)
-        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked);
     else
-        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
+        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked);
 }
 
 void launch_batch_finalize(sgk_batch *b) {
