@@ -59,16 +59,28 @@ def calibrate(gib=1.0, reps=5):
                       "bytes_written_per_launch": 4 * n, "launches": reps}))
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_force launch from the committed rocprofv3 --pmc summary of this workload
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_force_{workload}_traffic.md")
+def profile_figure(name, key):
+    """A figure recorded in a committed rocprofv3 summary (profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_force launch from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or None."""
+    return profile_figure(f"k_force_{workload}_traffic.md", "traffic_bytes_per_launch")
+
+
+def valu_issue_us(workload):
+    """Lower bound on the SIMD time of one k_force launch (mixed form): its VALU wave-instructions
+    (SQ_INSTS_VALU, committed rocprofv3 summary) x 4 cycles / (1,024 SIMDs x 2.4 GHz), or None."""
+    n = profile_figure(f"k_force_{workload}_valu.md", "valu_instructions_per_launch")
+    return None if n is None else n * 4.0 / (1024 * 2.4e9) * 1e6
 
 
 def nve_drift(config, steps, oracle_steps):
@@ -451,7 +463,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config),
                      "kernel": "k_force (fused metal+carbon force pass)", "us_per_launch": force_us,
-                     "alg_bytes_per_atom_step": b_alg("mixed", f_me)},
+                     "alg_bytes_per_atom_step": b_alg("mixed", f_me),
+                     # the pass is instruction-bound: share of its launch time the SIMDs spend issuing VALU
+                     "valu_issue_frac": (valu_issue_us(args.config) / force_us
+                                         if valu_issue_us(args.config) is not None else None)},
         "kernels": kernels,
     }
     if not args.no_check:
