@@ -72,16 +72,24 @@ __device__ __forceinline__ void rho_body(const DevParams &p, int blk, const type
         const P4 A = pos[i];
         const int nn = nnM[i];
         const int *row = nlM + (size_t)i * p.nlCap;
+        int kn[4];  // the next round's list entries, loaded a round ahead of their position gathers
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = l + q * G;
+            kn[q] = j < nn ? row[j] : i;
+        }
         for (int base = l; base < nn; base += G * 4) {
             int k[4];
             P4 B[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int j = base + q * G;
-                k[q] = j < nn ? row[j] : i;
-            }
+            for (int q = 0; q < 4; ++q) k[q] = kn[q];
 #pragma unroll
             for (int q = 0; q < 4; ++q) B[q] = pos[k[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = base + G * 4 + q * G;
+                kn[q] = j < nn ? row[j] : i;
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (k[q] >= p.nMolMe || base + q * G >= nn) continue;
