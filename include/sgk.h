@@ -64,8 +64,9 @@ typedef struct sgk_config {
                              pass - give each of the 8 XCDs a contiguous 1/8 of the atoms */
     int bin_group;        /* binning: one atomicAdd per distinct cell of a wave instead of per atom */
     int nl_scheme;        /* neighbour list: 0 one thread per atom, 1 one wave per occupied cell,
-                             2 LDS-staged per workgroup, 3 lanes per metal atom and carbons in cell
-                             order (sgk_build_nl_order); every scheme gives the same lists */
+                             2 LDS-staged per workgroup, 3 (default) lanes per metal atom, atoms
+                             visited in cell order (sgk_build_nl_order); every scheme gives the same
+                             lists */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with
@@ -244,7 +245,8 @@ int sgk_adhesion_energy(sgk_ctx *ctx, double *e);
  * time step, neighbour-list shell and capacity, thermostat schedule and RDF bins (checked at
  * creation); geometry, atom counts and temperature are free.  Each replica keeps its own cells,
  * lists, step state, reductions and records, and its results equal those of the same replica run
- * alone in an sgk_ctx with xcd_map = 0 and the same lanes per metal atom. */
+ * alone in an sgk_ctx with xcd_map = 0 and the same lanes per metal atom.  The list build is the
+ * LDS-staged one for nl_scheme 2 and the per-atom one otherwise (the lists are the same). */
 typedef struct sgk_batch sgk_batch;
 int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfg, sgk_batch **out);
 int sgk_batch_destroy(sgk_batch *b);
