@@ -627,10 +627,10 @@ __global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>
 //   carbon part (blocks [nbM, nbM + nbC)): one lane per carbon.
 //   Both visit their atoms in the order of their cells (nlOrder, built by build_nl_order; index
 //   order without one), so the lanes of a wave hold a few adjacent cells that share their neighbour
-//   cells and their candidate loads hit the same cache lines.
-//               The walk has no per-candidate bound test (positions padded with kFarAway to a
-//               multiple of 8) and no self test outside the atom's own cell unless a short grid axis
-//               wraps a neighbour offset onto it.
+//   cells and their candidate loads hit the same cache lines.  The carbon walk goes in chunks of 4
+//   and the metal one in chunks of 8; neither has a per-candidate bound test (positions padded with
+//   kFarAway to a multiple of 8), nor a self test outside the atom's own cell unless a short grid
+//   axis wraps a neighbour offset onto it.
 constexpr int kNlMetalLanes = 4;  // each takes 2 of every 8 candidates (8 = the slot padding)
 
 // Lists of metal atom i, G lanes per atom (l = lane within the group); called by whole groups.  Per
@@ -704,22 +704,22 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
                   fullNN, st);
 }
 
-// One chunk of 8 candidates of a carbon atom's walk (SELF: the cell may hold the atom itself).
+// One chunk of 4 candidates of a carbon atom's walk (SELF: the cell may hold the atom itself).  A
+// sheet cell holds ~10 atoms: chunks of 4 walk 12 slots for it where chunks of 8 walk 16.
 template <bool SELF, bool KEEP>
 __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, const float3 &A, float shx, float shy,
                                           float shz, const int *__restrict__ s, const float4 *__restrict__ cp,
                                           int &nn, int &nljv, int &nclv, int &err, unsigned *__restrict__ lj,
                                           unsigned *__restrict__ cl, int *__restrict__ fullNBL) {
     const int4 k0 = *(const int4 *)s;
-    const int4 k1 = *(const int4 *)(s + 4);
-    float4 P[8];
+    float4 P[4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) P[t] = cp[t];
-    const int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+    for (int t = 0; t < 4; ++t) P[t] = cp[t];
+    const int k[4] = {k0.x, k0.y, k0.z, k0.w};
     const float rrN = p.rrNebr, rrC = p.closeR2;
     const int nMe = p.nMolMe;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < 4; ++t) {
         const float rr = shifted_rr(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz);
         bool hit = rr < rrN;
         if (SELF) hit = hit && k[t] != i;
@@ -774,11 +774,11 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
         const int *s = slots + (size_t)C * p.cellCap;
         const float4 *cp = cpos + (size_t)C * p.cellCap;
         if (o == 0 || wrapSelf) {
-            for (int base = 0; base < n; base += 8)
+            for (int base = 0; base < n; base += 4)
                 nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
                                       fullNBL);
         } else {
-            for (int base = 0; base < n; base += 8)
+            for (int base = 0; base < n; base += 4)
                 nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
                                        fullNBL);
         }
