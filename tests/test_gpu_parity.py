@@ -69,8 +69,8 @@ def check_lists(p, prec, scheme, r, ordered=False):
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 @pytest.mark.parametrize("scheme", [3, 2, 1, 0])
 def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
-    """Every list scheme (owned cells with lanes per metal atom, LDS-staged per workgroup, one wave per
-    cell, one thread per atom) gives the reference list."""
+    """Every list scheme (lanes per metal atom and a cell-ordered walk, LDS-staged per workgroup, one
+    wave per cell, one thread per atom) gives the reference list."""
     s = system(name)
     print(f"[parity] cells/NL {name} {prec} scheme {scheme}", flush=True)
     check_lists(s.params, prec, scheme, s.r)
@@ -79,7 +79,7 @@ def test_cells_and_neighbour_list_bit_exact(gpu, name, prec, scheme):
 @pytest.mark.parametrize("name", ["small", "icosa", "medium"])
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_cell_ordered_neighbour_list_bit_exact(gpu, name, prec):
-    """nl_scheme 3 with its carbon order built from the same positions."""
+    """nl_scheme 3 with its list-build order made from the same positions."""
     s = system(name)
     print(f"[parity] cells/NL {name} {prec} scheme 3, cell order", flush=True)
     check_lists(s.params, prec, 3, s.r, ordered=True)
@@ -359,6 +359,46 @@ def test_autotune_keeps_results(gpu):
         b = np.array([x["totEnergy"] for x in recs_ref])
         # the XCD map regroups the block partials: FP summation order of the energy sums only
         assert np.abs(a - b).max() <= (1e-9 if prec == "mixed" else 1e-6) * np.abs(b).max()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_device_loop_shear(gpu, prec):
+    """Regime 2 past stepEquil + stepCool: ApplyShearK's force on the metal atoms behind the cluster's
+    centre of mass, raised by deltaF on every step the cluster's x velocity stays below 0.005
+    (SurfaceGrowth.cu:2398-2414), inside the device-resident loop, against the oracle's loop.  On
+    these steps the force pass must leave the next half-kick to k_lf1 (the shear changes the
+    accelerations after it)."""
+    s = sg.make_system(regime=2, material=3, ux=5, uy=6, n_metal=40, temperature=300.0, step_equil=5,
+                       step_cool=10, delta_f=0.001, dt=0.005)
+    p = s.params
+    n = 60
+    o = oracle.Oracle(p, "f64" if prec == "mixed" else "f32")
+    err, r_o, v_o, a_o, orecs = o.run(s.r, s.v, s.a, n)
+    assert err == 0
+    print(f"[parity] shear {prec}", flush=True)
+    with sg.Engine(p, precision=prec) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.set_run_state(0, 1, 0.0)
+        recs = eng.run(n)
+        r_g, v_g, _ = eng.download()
+    # the shear sequence the oracle's records imply
+    want, sh = [], 0.0
+    for x in orecs:
+        if x["step"] > p.stepEquil + p.stepCool and x["cmVelSum"][0] / p.nMolMe < 0.005:
+            sh += float(np.float32(p.deltaF))
+        want.append(sh)
+    got = [x["shear"] for x in recs]
+    assert want[-1] > 0
+    # the mixed form accumulates in FP64, as this sum; the plain form in FP32 (45 additions)
+    assert np.allclose(got, want, rtol=1e-12 if prec == "mixed" else 1e-5, atol=0), (got, want)
+    assert [x["rebuilt"] for x in recs] == [x["rebuilt"] for x in orecs]
+    tol = 1e-6 if prec == "mixed" else 2e-4
+    for k in ("totEnergy", "potEnergy", "kinEnergy"):
+        a = np.array([x[k] for x in recs])
+        b = np.array([x[k] for x in orecs])
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), k
+    assert np.abs(r_g[:, :3] - r_o[:, :3]).max() < (1e-6 if prec == "mixed" else 1e-3)
+    assert np.abs(v_g - v_o).max() < (1e-6 if prec == "mixed" else 1e-3) * np.abs(v_o).max()
 
 
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
