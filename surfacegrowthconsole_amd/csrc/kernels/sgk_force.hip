@@ -205,22 +205,26 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
         const T dFi = (T)embDF[i];
         const int nn = nnM[i];
         const int *row = nlM + (size_t)i * p.nlCap;
-        for (int base = l; base < nn; base += G * 4) {
-            int k[4];
-            P4 B[4];
-            float dFk[4];
+        // Gathers in flight per lane: 4 with FP32 pair math; 2 with FP64, whose pair temporaries
+        // would otherwise push the kernel past 128 VGPRs (3 waves per SIMD instead of 4).  Lane l
+        // takes entries l, l + G, l + 2G, ... in order either way, so the sums do not change.
+        constexpr int MC = sizeof(T) == 8 ? 2 : 4;
+        for (int base = l; base < nn; base += G * MC) {
+            int k[MC];
+            P4 B[MC];
+            float dFk[MC];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < MC; ++q) {
                 const int j = base + q * G;
                 k[q] = j < nn ? row[j] : i;
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < MC; ++q) {
                 B[q] = pos[k[q]];
                 dFk[q] = k[q] < p.nMolMe ? embDF[k[q]] : 0.f;
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < MC; ++q) {
                 if (base + q * G >= nn) continue;
                 R dxr, dyr, dzr;
                 const R rrr = disp<R>(Ai, B[q], p, dxr, dyr, dzr);
@@ -406,20 +410,32 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
 #pragma unroll
     for (int b = 0; b < 3; ++b) {  // unrolled: every array index is a constant, nothing goes to scratch
         if (b >= counti) break;
+        // The two partners are picked before any arithmetic: where i sits in k's list differs from
+        // lane to lane, and a loop over all three entries would issue three evaluations per bond.
         const unsigned me[3] = {nbk[b].x, nbk[b].y, nbk[b].z};
         const int na = (int)nbk[b].w;
-        P4 Bm[3];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) Bm[t] = pos[t < na ? (int)(me[t] >> 8) : i];
-        int countk = 0;
+        int ms[2] = {i, i}, nsel = 0;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            if (t >= na || countk >= 2 || (int)(me[t] >> 8) == i) continue;
+            const int m = (int)(me[t] >> 8);
+            if (t < na && m != i) {
+                if (nsel == 0)
+                    ms[0] = m;
+                else if (nsel == 1)
+                    ms[1] = m;
+                ++nsel;
+            }
+        }
+        P4 Bm[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) Bm[t] = pos[ms[t]];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (t >= nsel) break;
             V3<R> e;
             const R rr2 = disp<R>(Bk[b], Bm[t], p, e.x, e.y, e.z);
             const V3<R> em = {-e.x, -e.y, -e.z};
             neighbour_angle<R>(p, rnn[b], absR[b], em, rr2, (float)(Bm[t].z - z0), fx, fy, fz);
-            ++countk;
         }
     }
     if (counti == 3) {  // SurfaceGrowth.cu:985-1061
@@ -521,19 +537,23 @@ __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, 
                                             const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
                                             int integrate) {
     using A = typename Prec<M>::A;
-    Partial<A> q;
-    if (b < nbM)
-        q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF, fric,
-                                integrate);
-    else
-        q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, vel, acc, lj, nlj, bondC, bondA,
-                              integrate);
-    const Partial<A> r = b < nbM ? block_reduce_partial<true>(q) : block_reduce_partial<false>(q);
-    if (threadIdx.x == 0) part[b] = r;
+    // Each branch reduces and stores on its own: a Partial merged across the two branches keeps
+    // both bodies' registers live at once (154 VGPRs in the FP64 form against 120 apart).
+    if (b < nbM) {
+        const Partial<A> q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF,
+                                                 fric, integrate);
+        const Partial<A> r = block_reduce_partial<true>(q);
+        if (threadIdx.x == 0) part[b] = r;
+    } else {
+        const Partial<A> q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, vel, acc, lj, nlj,
+                                               bondC, bondA, integrate);
+        const Partial<A> r = block_reduce_partial<false>(q);
+        if (threadIdx.x == 0) part[b] = r;
+    }
 }
 
 template <bool M, class T, int G>
-__global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
                                                typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
                                                const int *__restrict__ nlM, const int *__restrict__ nnM,
                                                const float *__restrict__ embF, const float *__restrict__ embDF,
@@ -548,7 +568,7 @@ __global__ __launch_bounds__(256) void k_force(DevParams p, int nbM, typename Pr
 
 // Batched: each replica's workgroups write into its own range of partials (kb_finalize reduces).
 template <bool M, class T, int G>
-__global__ __launch_bounds__(256) void kb_force(BatchArgs a, int integrate) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_force(BatchArgs a, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
     const int2 e = a.forceTab[blockIdx.x];
