@@ -39,20 +39,20 @@ __device__ __forceinline__ R disp(const P4 &a, const P4 &b, const DevParams &p, 
 
 constexpr int kChunk = 8;  // neighbours gathered per round
 
-// Eight consecutive list entries (two 16-byte loads; rows are 32-byte aligned); entries at or past
-// `valid` are replaced by `pad` so they index a real atom.
-__device__ __forceinline__ void load_chunk(const unsigned *row, int valid, unsigned pad, unsigned (&ent)[kChunk]) {
-    const uint4 a = *(const uint4 *)row, b = *(const uint4 *)(row + 4);
-    ent[0] = a.x;
-    ent[1] = a.y;
-    ent[2] = a.z;
-    ent[3] = a.w;
-    ent[4] = b.x;
-    ent[5] = b.y;
-    ent[6] = b.z;
-    ent[7] = b.w;
+// N consecutive list entries (N a multiple of 4, 16-byte loads; rows are 32-byte aligned); entries
+// at or past `valid` are replaced by `pad` so they index a real atom.
+template <int N>
+__device__ __forceinline__ void load_chunk(const unsigned *row, int valid, unsigned pad, unsigned (&ent)[N]) {
 #pragma unroll
-    for (int t = 0; t < kChunk; ++t)
+    for (int h = 0; h < N / 4; ++h) {
+        const uint4 v = *(const uint4 *)(row + 4 * h);
+        ent[4 * h] = v.x;
+        ent[4 * h + 1] = v.y;
+        ent[4 * h + 2] = v.z;
+        ent[4 * h + 3] = v.w;
+    }
+#pragma unroll
+    for (int t = 0; t < N; ++t)
         if (t >= valid) ent[t] = pad;
 }
 
@@ -72,26 +72,29 @@ __device__ __forceinline__ void rho_body(const DevParams &p, int blk, const type
         const P4 A = pos[i];
         const int nn = nnM[i];
         const int *row = nlM + (size_t)i * p.nlCap;
-        int kn[4];  // the next round's list entries, loaded a round ahead of their position gathers
+        // gathers per round: 2 with FP64 pair math (88 VGPRs, 5 waves per SIMD), 4 with FP32; lane l
+        // sums entries l, l + G, l + 2G, ... in order either way
+        constexpr int MC = sizeof(T) == 8 ? 2 : 4;
+        int kn[MC];  // the next round's list entries, loaded a round ahead of their position gathers
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < MC; ++q) {
             const int j = l + q * G;
             kn[q] = j < nn ? row[j] : i;
         }
-        for (int base = l; base < nn; base += G * 4) {
-            int k[4];
-            P4 B[4];
+        for (int base = l; base < nn; base += G * MC) {
+            int k[MC];
+            P4 B[MC];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) k[q] = kn[q];
+            for (int q = 0; q < MC; ++q) k[q] = kn[q];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) B[q] = pos[k[q]];
+            for (int q = 0; q < MC; ++q) B[q] = pos[k[q]];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int j = base + G * 4 + q * G;
+            for (int q = 0; q < MC; ++q) {
+                const int j = base + G * MC + q * G;
                 kn[q] = j < nn ? row[j] : i;
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < MC; ++q) {
                 if (k[q] >= p.nMolMe || base + q * G >= nn) continue;
                 R dx, dy, dz;
                 const R rr = disp<R>(A, B[q], p, dx, dy, dz);
@@ -127,14 +130,16 @@ __device__ __forceinline__ void bonds_body(const DevParams &p, int blk, const ty
     unsigned cb[3] = {0u, 0u, 0u}, ab[3] = {0u, 0u, 0u};
     int nc = 0, na = 0;
     const int nclose = ncl[ic];
-    for (int base = 0; base < nclose && (nc < 3 || na < 3); base += kChunk) {
-        unsigned ent[kChunk];
-        P4 B[kChunk];
+    // 4 entries a round with FP64 positions, 8 with FP32 (register pressure of the density launch)
+    constexpr int CB = sizeof(R) == 8 ? 4 : kChunk;
+    for (int base = 0; base < nclose && (nc < 3 || na < 3); base += CB) {
+        unsigned ent[CB];
+        P4 B[CB];
         load_chunk(cl + (size_t)ic * p.clCap + base, nclose - base, (unsigned)i << 8, ent);
 #pragma unroll
-        for (int t = 0; t < kChunk; ++t) B[t] = pos[ent[t] >> 8];
+        for (int t = 0; t < CB; ++t) B[t] = pos[ent[t] >> 8];
 #pragma unroll
-        for (int t = 0; t < kChunk; ++t) {
+        for (int t = 0; t < CB; ++t) {
             if (base + t >= nclose) continue;
             R dx, dy, dz;
             const R rr = disp<R>(Ai, B[t], p, dx, dy, dz);
