@@ -40,12 +40,6 @@ __device__ __forceinline__ R wrap1(R d, R L) {
     return d;
 }
 
-template <class P4, class R>
-__device__ __forceinline__ float3 min_image(const P4 &a, const P4 &b, R Lx, R Ly, R Lz) {
-    return make_float3((float)wrap1<R>(a.x - b.x, Lx), (float)wrap1<R>(a.y - b.y, Ly),
-                       (float)wrap1<R>(a.z - b.z, Lz));
-}
-
 // gfx950 dispatches workgroups to its 8 XCDs round robin: workgroup b runs on XCD b % 8, and each
 // XCD has its own L2.  xcd_block maps the hardware index b of a range of n workgroups to a logical
 // index so that every XCD works on one contiguous 1/8 of the range.  Atoms with nearby indices are
