@@ -123,11 +123,12 @@ __device__ __forceinline__ void sort_cells(const DevParams &p, const CellBufs &b
         if (idx < cnt) {
             s[idx] = v;
             const float3 q = f3(pos[v]);
-            bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(q.x, q.y, q.z, 0.f);
+            bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(q.x, q.y, q.z, __int_as_float(v));
         } else if (idx < ((cnt + 7) & ~7)) {
             // padding up to the next multiple of 8: a position no list radius reaches, so a walk in
             // chunks of 8 needs no per-candidate bound test
-            bufs.cellPos[(size_t)c * p.cellCap + idx] = make_float4(kFarAway, kFarAway, kFarAway, 0.f);
+            bufs.cellPos[(size_t)c * p.cellCap + idx] =
+                make_float4(kFarAway, kFarAway, kFarAway, __int_as_float(0x7fffffff));
         }
     }
 }
@@ -237,7 +238,6 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
     if (i >= p.nMol) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
     const int *cnt = bufs.cnt[b];
-    const int *slots = bufs.slots[b];
     const float4 *cpos = bufs.cellPos;
     const bool metal = i < p.nMolMe;
     const int ic = i - p.nMolMe;
@@ -260,16 +260,14 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
                 nnext = cnt[Cn];
             }
             const int n = min(ncur, p.cellCap);
-            const int *s = slots + (size_t)C * p.cellCap;
             const float4 *cp = cpos + (size_t)C * p.cellCap;
             for (int base = 0; base < n; base += 8) {
-                // slot indices and their positions are independent loads (cell-ordered copy)
-                const int4 k0 = *(const int4 *)(s + base);
-                const int4 k1 = *(const int4 *)(s + base + 4);
                 float4 P[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) P[t] = cp[base + t];
-                const int k[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+                int k[8];  // the slot's atom index rides in w
+#pragma unroll
+                for (int t = 0; t < 8; ++t) k[t] = __float_as_int(P[t].w);
                 float3 B[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) B[t] = make_float3(P[t].x, P[t].y, P[t].z);
@@ -375,7 +373,6 @@ __device__ __forceinline__ void nlt_body(const DevParams &p, int blk, const type
     const int BX = sHi[0] - sLo[0] + 3, BY = sHi[1] - sLo[1] + 3, BZ = sHi[2] - sLo[2] + 3;
     const long long Hl = (long long)BX * BY * BZ;
     const int *cnt = bufs.cnt[b];
-    const int *slots = bufs.slots[b];
     // box cell h -> grid cell, with the reference's single wrap per axis (stencil_cell)
     auto grid_cell = [&](int h) {
         const int hz = h / (BX * BY), hy = (h / BX) % BY, hx = h % BX;
@@ -406,9 +403,7 @@ __device__ __forceinline__ void nlt_body(const DevParams &p, int blk, const type
             if (sStart[mid] <= e) lo = mid; else hi = mid;
         }
         const size_t g = (size_t)grid_cell(lo) * p.cellCap + (e - sStart[lo]);
-        float4 q = bufs.cellPos[g];
-        q.w = __int_as_float(slots[g]);
-        sEnt[e] = q;
+        sEnt[e] = bufs.cellPos[g];  // w holds the slot's atom index
     }
     __syncthreads();
     if (!in) return;
@@ -668,13 +663,12 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
                 nnext = cnt[Cn];
             }
             const int n = min(ncur, p.cellCap);
-            const int *s = bufs.slots[b] + (size_t)C * p.cellCap;
             const float4 *cp = bufs.cellPos + (size_t)C * p.cellCap;
             const bool self = o == 0 || wrapSelf;
             for (int base = 0; base < n; base += 2 * G) {
                 const int t0 = base + l, t1 = base + G + l;  // < n rounded up to 8
-                const int k0 = s[t0], k1 = s[t1];
                 const float4 q0 = cp[t0], q1 = cp[t1];
+                const int k0 = __float_as_int(q0.w), k1 = __float_as_int(q1.w);
                 const float rr0 = shifted_rr(A, make_float3(q0.x, q0.y, q0.z), shx, shy, shz);
                 const float rr1 = shifted_rr(A, make_float3(q1.x, q1.y, q1.z), shx, shy, shz);
                 const bool h0 = rr0 < rrN && (!self || k0 != i);
@@ -708,14 +702,15 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
 // sheet cell holds ~10 atoms: chunks of 4 walk 12 slots for it where chunks of 8 walk 16.
 template <bool SELF, bool KEEP>
 __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, const float3 &A, float shx, float shy,
-                                          float shz, const int *__restrict__ s, const float4 *__restrict__ cp,
-                                          int &nn, int &nljv, int &nclv, int &err, unsigned *__restrict__ lj,
-                                          unsigned *__restrict__ cl, int *__restrict__ fullNBL) {
-    const int4 k0 = *(const int4 *)s;
+                                          float shz, const float4 *__restrict__ cp, int &nn, int &nljv,
+                                          int &nclv, int &err, unsigned *__restrict__ lj, unsigned *__restrict__ cl,
+                                          int *__restrict__ fullNBL) {
     float4 P[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) P[t] = cp[t];
-    const int k[4] = {k0.x, k0.y, k0.z, k0.w};
+    int k[4];  // the slot's atom index rides in w
+#pragma unroll
+    for (int t = 0; t < 4; ++t) k[t] = __float_as_int(P[t].w);
     const float rrN = p.rrNebr, rrC = p.closeR2;
     const int nMe = p.nMolMe;
 #pragma unroll
@@ -750,7 +745,7 @@ __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, con
 // Lists of carbon atom i in cell c (one lane).
 template <bool KEEP>
 __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const float3 A, const int *__restrict__ cnt,
-                                         const int *__restrict__ slots, const float4 *__restrict__ cpos,
+                                         const float4 *__restrict__ cpos,
                                          DevState *st, unsigned *__restrict__ lj, int *__restrict__ nlj,
                                          unsigned *__restrict__ cl, int *__restrict__ ncl,
                                          int *__restrict__ fullNN, int *__restrict__ fullNBL) {
@@ -771,15 +766,14 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
             nnext = cnt[Cn];
         }
         const int n = min(ncur, p.cellCap);
-        const int *s = slots + (size_t)C * p.cellCap;
         const float4 *cp = cpos + (size_t)C * p.cellCap;
         if (o == 0 || wrapSelf) {
             for (int base = 0; base < n; base += 4)
-                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
+                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, cp + base, nn, nljv, nclv, err, lj, cl,
                                       fullNBL);
         } else {
             for (int base = 0; base < n; base += 4)
-                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, s + base, cp + base, nn, nljv, nclv, err, lj, cl,
+                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, cp + base, nn, nljv, nclv, err, lj, cl,
                                        fullNBL);
         }
         C = Cn;
@@ -807,7 +801,7 @@ __device__ __forceinline__ void nlc_body(const DevParams &p, int blk, const type
         nl_finish(p, i, false, i - p.nMolMe, 0, 0, 0, 0, nullptr, nlj, ncl, fullNN, st);
         return;
     }
-    nlc_walk<KEEP>(p, i, c, f3(pos[i]), bufs.cnt[b], bufs.slots[b], bufs.cellPos, st, lj, nlj, cl, ncl, fullNN,
+    nlc_walk<KEEP>(p, i, c, f3(pos[i]), bufs.cnt[b], bufs.cellPos, st, lj, nlj, cl, ncl, fullNN,
                    fullNBL);
 }
 

@@ -42,7 +42,8 @@ struct DevParams {
 
 // Two cell buffers; state.cellbuf names the current one (see the rebuild protocol in sgk_nebr.hip).
 // cellOf[b][n] is atom n's cell in buffer b (-1: not binned); slotOf[n] its slot in the buffer
-// being built; cellPos[c*cap + s] the float position of the atom in slot s of cell c after the sort
+// being built; cellPos[c*cap + s] the float position of the atom in slot s of cell c after the sort,
+// with the atom's index (its bit pattern) in w, so a list walk reads one 16-byte entry per candidate
 // (built with the buffer, read by the neighbour-list pass).
 struct CellBufs {
     int *cnt[2];
