@@ -61,7 +61,8 @@ typedef struct sgk_config {
                                  stepEquil < step < stepEquil + stepCool) */
     int pair_fp64;        /* mixed form only: EAM and LJ pair arithmetic in FP64 (default) or FP32 */
     int xcd_map;          /* launch shape: bit 0 neighbour list, bit 1 density/near lists, bit 2 force
-                             pass - give each of the 8 XCDs a contiguous 1/8 of the atoms */
+                             pass - give each of the 8 XCDs a contiguous 1/8 of the atoms (default 2;
+                             bits 0 and 1 leave every result unchanged) */
     int bin_group;        /* binning: one atomicAdd per distinct cell of a wave instead of per atom */
     int nl_scheme;        /* neighbour list: 0 one thread per atom, 1 one wave per occupied cell,
                              2 LDS-staged per workgroup, 3 (default) lanes per metal atom, atoms
@@ -187,8 +188,8 @@ int sgk_metal_lanes(const sgk_ctx *ctx);
 
 /* Autotuner (no reference counterpart: the reference has fixed launch shapes).  Times the density
  * and force passes on the current state for every candidate launch shape - lanes per metal atom
- * 1/2/4/8/16 and the XCD map of those passes on or off - `reps` launches each, and keeps the
- * fastest.  The passes only rewrite quantities they recompute identically (acceleration, per-atom
+ * 1/2/4/8/16 and the XCD map bits of the density and force passes (4 combinations, 20 shapes) -
+ * `reps` launches each, and keeps the fastest.  The passes only rewrite quantities they recompute identically (acceleration, per-atom
  * energy, friction), so the state is unchanged.  Needs a valid neighbour list.  Results differ
  * between shapes only by the FP summation order of the metal neighbour loop. */
 typedef struct sgk_tune_entry {

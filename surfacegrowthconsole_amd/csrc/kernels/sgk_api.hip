@@ -224,7 +224,8 @@ void sgk_default_config(sgk_config *cfg) {
     cfg->records_capacity = 1024;
     cfg->use_graphs = 1;
     cfg->pair_fp64 = 1;  // mixed form: FP64 pair arithmetic (measured cost nil, error < 1e-5)
-    cfg->xcd_map = 0;    // remap measured 1-3% slower per step on MI355X; kept as an option
+    cfg->xcd_map = 2;    // density pass remapped (57.7 -> 54.2 us at 1.1 M atoms); the force pass and
+                         // the list measured slower with it on MI355X
     cfg->bin_group = 1;
     cfg->nl_scheme = 3;  // 1.1 M atoms, thermalised: 0.35-0.37 ms per rebuild; per atom (0) 0.42 ms
 }
@@ -944,7 +945,8 @@ int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, 
     const bool prof = c->profiling;
     c->profiling = false;
     const int lanes[] = {1, 2, 4, 8, 16};
-    const int maps[] = {c->dp.xcdMap & ~6, c->dp.xcdMap | 6};
+    const int base = c->dp.xcdMap & ~6;  // the density and force bits are tuned, the list bit is kept
+    const int maps[] = {base | 2, base, base | 6, base | 4};
     int bestG = c->metalLanes, bestMap = c->dp.xcdMap, n = 0;
     double best = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -252,9 +252,9 @@ class Engine:
     def autotune(self, reps=20):
         """Pick the fastest launch shape of the density and force passes on the current state;
         returns [(metal_lanes, xcd_map, us), ...] for every candidate."""
-        buf = (N.SgkTuneEntry * 16)()
+        buf = (N.SgkTuneEntry * 32)()
         n = C.c_int()
-        self._check(self.lib.sgk_autotune(self.h, reps, buf, 16, C.byref(n)), "sgk_autotune")
+        self._check(self.lib.sgk_autotune(self.h, reps, buf, 32, C.byref(n)), "sgk_autotune")
         return [(e.metal_lanes, e.xcd_map, e.us) for e in buf[: n.value]]
 
     def kernel_stats(self):
