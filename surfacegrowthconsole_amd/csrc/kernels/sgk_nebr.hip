@@ -5,7 +5,8 @@
 //               (bin_atom; in the step loop it runs inside the leapfrog kernel).
 //   k_sortcells per atom: the atom that took slot 0 of its cell has its wave sort that cell's
 //               slots ascending (bitonic, in registers) and write the cell-ordered float
-//               positions (padded with a far-away point to a multiple of 8); every atom clears its
+//               positions with each atom's index in w (padded with a far-away point to a multiple
+//               of 8), so a list walk reads one 16-byte entry per candidate; every atom clears its
 //               cell's count in cur for the rebuild after this one.  No global counter is involved.
 //   k_nlc       the lists in the reference order from the sorted cells: lanes per metal atom, one
 //               lane per carbon in cell order (nl_scheme 3, the default; schemes 0-2: k_nl, k_nlt).
