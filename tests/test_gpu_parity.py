@@ -231,6 +231,51 @@ def test_forces_vs_oracle(gpu, name, state):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_bulk_metal_forces_across_periodic_faces(gpu, prec):
+    """Periodic bulk Ni (5x5x5 fcc cells, 500 atoms, a box 3 list radii wide) with every atom
+    displaced at random: nearly every atom lies within one list radius of a face, so its density and
+    force pairs take the single-image correction, which the pair displacement applies only when a component reaches half
+    the box.  Density and forces against the oracle at the same positions: the mixed form against
+    FP64 (e_F <= 1e-5, rho within 1e-6), the plain form against the FP32 restatement (rho within
+    1e-5; e_F <= 2e-5, since no angle term amplifies its rounding here)."""
+    s = sg.make_system(regime=0, material=3, ux=5, uy=5, uz=5, temperature=300.0, geometry="reference")
+    p = s.params
+    rng = np.random.default_rng(11)
+    r = s.r.astype(np.float64)
+    r[:, :3] += rng.uniform(-0.04, 0.04, (p.nMol, 3))
+    half = 0.5 * np.array([p.region.x, p.region.y, p.region.z])
+    near_face = (np.abs(r[:, :3]) > half - np.sqrt(p.rrNebr)).any(1)
+    assert near_face.mean() > 0.5
+    rin = r if prec == "mixed" else r.astype(np.float32)
+    o64 = oracle.Oracle(p, "f64")
+    err, nn, nbl = o64.neighbour_list(rin)
+    assert err == 0
+    rho64 = o64.rho(rin, nn, nbl)
+    a64, e64, _ = o64.forces(rin, nn, nbl, rho64)
+    print(f"[parity] bulk Ni {p.nMol} atoms {prec}", flush=True)
+    with sg.Engine(p, precision=prec) as eng:
+        eng.upload(rin)
+        eng.bin()
+        eng.build_nebr()
+        eng.compute_forces()
+        rho = eng.rho()
+        acc, e, _ = eng.forces()
+    if prec == "mixed":
+        ef = force_error(acc, a64)
+        assert np.abs(rho - rho64).max() / np.abs(rho64).max() < 1e-6
+        assert abs(e.sum() - e64.sum()) <= 1e-7 * np.abs(e64).sum()
+    else:
+        o32 = oracle.Oracle(p, "f32")
+        rho32 = o32.rho(rin, nn, nbl)
+        a32, e32, _ = o32.forces(rin, nn, nbl)
+        ef = np.abs(acc - a32).max() / np.sqrt((a64 ** 2).sum(1).mean())
+        assert np.abs(rho - rho32).max() / np.abs(rho32).max() < 1e-5
+        assert abs(e.sum() - e32.sum()) <= 1e-5 * np.abs(e32).sum()
+    print(f"[parity]   e_F = {ef:.3e}", flush=True)
+    assert ef < (1e-5 if prec == "mixed" else 2e-5), ef
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
 def test_integrator_thermostat_props(gpu, prec):
     s = system("small")
     p = s.params
