@@ -699,25 +699,23 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
                   fullNN, st);
 }
 
-// One chunk of 4 candidates of a carbon atom's walk (SELF: the cell may hold the atom itself).  A
-// sheet cell holds ~10 atoms: chunks of 4 walk 12 slots for it where chunks of 8 walk 16.
+// One chunk of 4 candidates of a carbon atom's column walk (SELF: the column may hold the atom
+// itself): candidates already loaded, each with its own z shift (the three cells of a column share
+// the x and y shift); t >= valid are not candidates.
 template <bool SELF, bool KEEP>
 __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, const float3 &A, float shx, float shy,
-                                          float shz, const float4 *__restrict__ cp, int &nn, int &nljv,
-                                          int &nclv, int &err, unsigned *__restrict__ lj, unsigned *__restrict__ cl,
-                                          int *__restrict__ fullNBL) {
-    float4 P[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) P[t] = cp[t];
-    int k[4];  // the slot's atom index rides in w
+                                          const float (&shz)[4], const float4 (&P)[4], int valid, int &nn,
+                                          int &nljv, int &nclv, int &err, unsigned *__restrict__ lj,
+                                          unsigned *__restrict__ cl, int *__restrict__ fullNBL) {
+    int k[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) k[t] = __float_as_int(P[t].w);
     const float rrN = p.rrNebr, rrC = p.closeR2;
     const int nMe = p.nMolMe;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const float rr = shifted_rr(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz);
-        bool hit = rr < rrN;
+        const float rr = shifted_rr(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz[t]);
+        bool hit = rr < rrN && t < valid;
         if (SELF) hit = hit && k[t] != i;
         if (hit && (k[t] < nMe || rr < rrC)) {
             if (nn < p.nlCap) {
@@ -756,32 +754,37 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
     const int ccx = c - (ccz * p.cy + ccy) * p.cx;
     const bool wrapSelf = p.cx < 3 || p.cy < 3 || p.cz < 3;
     int nn = 0, nljv = 0, nclv = 0, err = 0;
-    float shx, shy, shz;
-    int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
-    int ncur = cnt[C];
-    for (int o = 0; o < 27; ++o) {
-        float nshx = 0.f, nshy = 0.f, nshz = 0.f;
-        int Cn = C, nnext = 0;
-        if (o < 26) {  // prefetch the next cell's count
-            Cn = stencil_cell<float>(p, ccx, ccy, ccz, o + 1, nshx, nshy, nshz);
-            nnext = cnt[Cn];
-        }
-        const int n = min(ncur, p.cellCap);
-        const float4 *cp = cpos + (size_t)C * p.cellCap;
-        if (o == 0 || wrapSelf) {
-            for (int base = 0; base < n; base += 4)
-                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, cp + base, nn, nljv, nclv, err, lj, cl,
+    // The offsets 3q, 3q+1, 3q+2 of the table share their x and y components (the z component is
+    // o % 3), so the walk takes them as one column: its three cells back to back, in chunks of 4
+    // that may straddle cells.  A sheet split over two z layers then walks ~12 slots per column
+    // instead of two padded cells of 8.
+    const int cap = p.cellCap;
+    for (int q = 0; q < 9; ++q) {
+        float shx, shy, sz0, sz1, sz2, tx, ty;
+        const int C0 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q, shx, shy, sz0);
+        const int C1 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q + 1, tx, ty, sz1);
+        const int C2 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q + 2, tx, ty, sz2);
+        const int n0 = min(cnt[C0], cap), n1 = min(cnt[C1], cap), n2 = min(cnt[C2], cap);
+        const int n01 = n0 + n1, tot = n01 + n2;
+        const bool self = q == 0 || wrapSelf;
+        for (int base = 0; base < tot; base += 4) {
+            float4 P[4];
+            float shz[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int v = min(base + t, tot - 1);
+                const int cell = v < n0 ? C0 : (v < n01 ? C1 : C2);
+                const int idx = v < n0 ? v : (v < n01 ? v - n0 : v - n01);
+                shz[t] = v < n0 ? sz0 : (v < n01 ? sz1 : sz2);
+                P[t] = cpos[(size_t)cell * cap + idx];
+            }
+            if (self)
+                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl,
                                       fullNBL);
-        } else {
-            for (int base = 0; base < n; base += 4)
-                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, cp + base, nn, nljv, nclv, err, lj, cl,
+            else
+                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl,
                                        fullNBL);
         }
-        C = Cn;
-        ncur = nnext;
-        shx = nshx;
-        shy = nshy;
-        shz = nshz;
     }
     if (nn > p.nlCap) err |= SGK_FLAG_NL_OVERFLOW;
     nl_finish(p, i, false, ic, nn, nljv, nclv, err, nullptr, nlj, ncl, fullNN, st);
