@@ -623,14 +623,17 @@ __global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>
 //   carbon part (blocks [nbM, nbM + nbC)): one lane per carbon.
 //   Both visit their atoms in the order of their cells (nlOrder, built by build_nl_order; index
 //   order without one), so the lanes of a wave hold a few adjacent cells that share their neighbour
-//   cells and their candidate loads hit the same cache lines.  The carbon walk goes in chunks of 4
-//   and the metal one in chunks of 8; neither has a per-candidate bound test (positions padded with
-//   kFarAway to a multiple of 8), nor a self test outside the atom's own cell unless a short grid
-//   axis wraps a neighbour offset onto it.
-constexpr int kNlMetalLanes = 4;  // each takes 2 of every 8 candidates (8 = the slot padding)
+//   cells and their candidate loads hit the same cache lines.  Both walk the 27 cells as 9 columns:
+//   offsets 3q, 3q+1 and 3q+2 of the table differ only in z, so a column's three cells are taken
+//   back to back, the carbon walk in chunks of 4 and the metal one in chunks of 8 that may straddle
+//   cells.  A graphene column split over two z layers is ~12 candidates: 3 chunks, where cell by
+//   cell it took two cells padded to 8.  Each candidate is one 16-byte load (position and atom
+//   index); there is no self test outside the atom's own column unless a short grid axis wraps a
+//   neighbour offset onto it.
+constexpr int kNlMetalLanes = 4;  // each takes 2 of every chunk of 8 candidates
 
 // Lists of metal atom i, G lanes per atom (l = lane within the group); called by whole groups.  Per
-// group step the lanes test candidates base + l and base + G + l of the cell (base a multiple of 8).
+// group step the lanes test candidates base + l and base + G + l of the column (base a multiple of 8).
 template <bool M, int G, bool KEEP>
 __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const typename Prec<M>::P4 *__restrict__ pos,
                                          const CellBufs &bufs, int b, DevState *st, const int *__restrict__ order,
@@ -653,27 +656,30 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
         const float rrN = p.rrNebr;
         const int *cnt = bufs.cnt[b];
         const bool wrapSelf = p.cx < 3 || p.cy < 3 || p.cz < 3;
-        float shx, shy, shz;
-        int C = stencil_cell<float>(p, ccx, ccy, ccz, 0, shx, shy, shz);
-        int ncur = cnt[C];
-        for (int o = 0; o < 27; ++o) {
-            float nshx = 0.f, nshy = 0.f, nshz = 0.f;
-            int Cn = C, nnext = 0;
-            if (o < 26) {  // prefetch the next cell's count
-                Cn = stencil_cell<float>(p, ccx, ccy, ccz, o + 1, nshx, nshy, nshz);
-                nnext = cnt[Cn];
-            }
-            const int n = min(ncur, p.cellCap);
-            const float4 *cp = bufs.cellPos + (size_t)C * p.cellCap;
-            const bool self = o == 0 || wrapSelf;
-            for (int base = 0; base < n; base += 2 * G) {
-                const int t0 = base + l, t1 = base + G + l;  // < n rounded up to 8
-                const float4 q0 = cp[t0], q1 = cp[t1];
+        // one column (offsets 3q, 3q+1, 3q+2: same x and y shift) at a time, its cells back to back
+        const int cap = p.cellCap;
+        const float4 *cpos = bufs.cellPos;
+        for (int q = 0; q < 9; ++q) {
+            float shx, shy, sz0, sz1, sz2, tx, ty;
+            const int C0 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q, shx, shy, sz0);
+            const int C1 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q + 1, tx, ty, sz1);
+            const int C2 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q + 2, tx, ty, sz2);
+            const int n0 = min(cnt[C0], cap), n1 = min(cnt[C1], cap), n2 = min(cnt[C2], cap);
+            const int n01 = n0 + n1, tot = n01 + n2;
+            const bool self = q == 0 || wrapSelf;
+            for (int base = 0; base < tot; base += 2 * G) {
+                const int t0 = base + l, t1 = base + G + l;
+                const int v0 = min(t0, tot - 1), v1 = min(t1, tot - 1);
+                const int c0 = v0 < n0 ? C0 : (v0 < n01 ? C1 : C2), c1 = v1 < n0 ? C0 : (v1 < n01 ? C1 : C2);
+                const int s0 = v0 < n0 ? v0 : (v0 < n01 ? v0 - n0 : v0 - n01);
+                const int s1 = v1 < n0 ? v1 : (v1 < n01 ? v1 - n0 : v1 - n01);
+                const float z0 = v0 < n0 ? sz0 : (v0 < n01 ? sz1 : sz2), z1 = v1 < n0 ? sz0 : (v1 < n01 ? sz1 : sz2);
+                const float4 q0 = cpos[(size_t)c0 * cap + s0], q1 = cpos[(size_t)c1 * cap + s1];
                 const int k0 = __float_as_int(q0.w), k1 = __float_as_int(q1.w);
-                const float rr0 = shifted_rr(A, make_float3(q0.x, q0.y, q0.z), shx, shy, shz);
-                const float rr1 = shifted_rr(A, make_float3(q1.x, q1.y, q1.z), shx, shy, shz);
-                const bool h0 = rr0 < rrN && (!self || k0 != i);
-                const bool h1 = rr1 < rrN && (!self || k1 != i);
+                const float rr0 = shifted_rr(A, make_float3(q0.x, q0.y, q0.z), shx, shy, z0);
+                const float rr1 = shifted_rr(A, make_float3(q1.x, q1.y, q1.z), shx, shy, z1);
+                const bool h0 = t0 < tot && rr0 < rrN && (!self || k0 != i);
+                const bool h1 = t1 < tot && rr1 < rrN && (!self || k1 != i);
                 const unsigned g0 = (unsigned)(__ballot(h0) >> gshift) & gmask;
                 const unsigned g1 = (unsigned)(__ballot(h1) >> gshift) & gmask;
                 const int at0 = nn + __popc(g0 & below), at1 = nn + __popc(g0) + __popc(g1 & below);
@@ -687,11 +693,6 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
                 }
                 nn += __popc(g0) + __popc(g1);
             }
-            C = Cn;
-            ncur = nnext;
-            shx = nshx;
-            shy = nshy;
-            shz = nshz;
         }
     }
     if (active && l == 0)
