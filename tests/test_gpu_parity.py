@@ -99,13 +99,18 @@ def test_neighbour_list_bit_exact_thermalised(gpu, name, scheme):
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
 @pytest.mark.parametrize("scheme", [3, 0])
-def test_neighbour_list_bit_exact_wrapped_grid(gpu, prec, scheme):
-    """32 Ni atoms in a 1 x 1 x 1 cell grid: all 27 offsets wrap onto the atom's own cell with a
-    shift, so every offset holds the atom itself and the other atoms' periodic images."""
-    s = sg.make_system(regime=0, material=3, ux=2, uy=2, uz=2, temperature=600.0, velocities="maxwell", seed=2)
+@pytest.mark.parametrize("dims,grid", [((2, 2, 2), (1, 1, 1)), ((6, 6, 2), (3, 3, 1)), ((2, 6, 6), (1, 3, 3)),
+                                       ((4, 4, 4), (2, 2, 2))])
+def test_neighbour_list_bit_exact_wrapped_grid(gpu, prec, scheme, dims, grid):
+    """Bulk Ni in cell grids with an axis shorter than 3 cells.  With 1 x 1 x 1 cells all 27 offsets
+    wrap onto the atom's own cell with a shift, so every offset holds the atom itself and the other
+    atoms' periodic images; with one z layer the three cells of every column are one cell seen with
+    z shifts 0, +Lz and -Lz; with 2 cells an offset of +1 and -1 reach the same cell."""
+    s = sg.make_system(regime=0, material=3, ux=dims[0], uy=dims[1], uz=dims[2], temperature=600.0,
+                       velocities="maxwell", seed=2)
     p = s.params
-    assert (p.cells.x, p.cells.y, p.cells.z) == (1, 1, 1)
-    print(f"[parity] NL wrapped grid {prec} scheme {scheme}", flush=True)
+    assert (p.cells.x, p.cells.y, p.cells.z) == grid
+    print(f"[parity] NL wrapped grid {grid} {prec} scheme {scheme}", flush=True)
     check_lists(p, prec, scheme, s.r)
 
 
