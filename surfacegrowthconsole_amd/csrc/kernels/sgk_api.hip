@@ -263,6 +263,8 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         return fail(nullptr, SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64 (one wave per cell)");
     if (cfg.close_capacity % 8) return fail(nullptr, SGK_E_UNSUPPORTED, "close capacity must be a multiple of 8");
     if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return fail(nullptr, SGK_E_ARG, "empty cell grid");
+    if ((long long)sp->cells.x * sp->cells.y * sp->cells.z * cfg.cell_capacity >= (1ll << 31))
+        return fail(nullptr, SGK_E_UNSUPPORTED, "cell grid too large: cells x cell capacity must stay below 2^31");
     if (sp->iRegime < 0 || sp->iRegime > 2) return fail(nullptr, SGK_E_ARG, "regime must be 0, 1 or 2");
     if (sp->sizeHistRdf <= 0 || sp->sizeHistRdf > 8192) return fail(nullptr, SGK_E_ARG, "RDF bins must be 1..8192");
     int ndev = 0;

@@ -767,6 +767,9 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
         const int C2 = stencil_cell<float>(p, ccx, ccy, ccz, 3 * q + 2, tx, ty, sz2);
         const int n0 = min(cnt[C0], cap), n1 = min(cnt[C1], cap), n2 = min(cnt[C2], cap);
         const int n01 = n0 + n1, tot = n01 + n2;
+        // candidate v of the column is entry v + o_k of the cell-ordered positions (k its cell);
+        // indices fit 32 bits (ncell * cellCap < 2^31 is checked at set-up)
+        const int o0 = C0 * cap, o1 = C1 * cap - n0, o2 = C2 * cap - n01;
         const bool self = q == 0 || wrapSelf;
         for (int base = 0; base < tot; base += 4) {
             float4 P[4];
@@ -774,10 +777,10 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int v = min(base + t, tot - 1);
-                const int cell = v < n0 ? C0 : (v < n01 ? C1 : C2);
-                const int idx = v < n0 ? v : (v < n01 ? v - n0 : v - n01);
-                shz[t] = v < n0 ? sz0 : (v < n01 ? sz1 : sz2);
-                P[t] = cpos[(size_t)cell * cap + idx];
+                const bool a0 = v < n0, a1 = v < n01;
+                const int o = a0 ? o0 : (a1 ? o1 : o2);
+                shz[t] = a0 ? sz0 : (a1 ? sz1 : sz2);
+                P[t] = cpos[(unsigned)(o + v)];
             }
             if (self)
                 nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl,
