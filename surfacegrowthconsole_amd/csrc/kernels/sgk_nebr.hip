@@ -66,13 +66,17 @@ __device__ __forceinline__ float3 f3(const P4 &q) {
 }
 
 // Distance test of the neighbour-list build and RDF: ((A - B) - shift)^2 in FP32 with every
-// operation rounded separately, the canonical evaluation the oracle uses.
+// operation rounded separately, the canonical evaluation the oracle uses.  SHIFT = false leaves out
+// the subtraction of shifts known to be zero (x - 0 is x exactly).
+template <bool SHIFT = true>
 __device__ __forceinline__ float shifted_rr(const float3 &a, const float3 &b, float shx, float shy, float shz) {
 #pragma clang fp contract(off)
     float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    dx = dx - shx;
-    dy = dy - shy;
-    dz = dz - shz;
+    if (SHIFT) {
+        dx = dx - shx;
+        dy = dy - shy;
+        dz = dz - shz;
+    }
     return dx * dx + dy * dy + dz * dz;
 }
 
@@ -703,7 +707,7 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
 // One chunk of 4 candidates of a carbon atom's column walk (SELF: the column may hold the atom
 // itself): candidates already loaded, each with its own z shift (the three cells of a column share
 // the x and y shift); t >= valid are not candidates.
-template <bool SELF, bool KEEP>
+template <bool SELF, bool KEEP, bool SHIFT>
 __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, const float3 &A, float shx, float shy,
                                           const float (&shz)[4], const float4 (&P)[4], int valid, int &nn,
                                           int &nljv, int &nclv, int &err, unsigned *__restrict__ lj,
@@ -715,7 +719,7 @@ __device__ __forceinline__ void nlc_chunk(const DevParams &p, int i, int ic, con
     const int nMe = p.nMolMe;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const float rr = shifted_rr(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz[t]);
+        const float rr = shifted_rr<SHIFT>(A, make_float3(P[t].x, P[t].y, P[t].z), shx, shy, shz[t]);
         bool hit = rr < rrN && t < valid;
         if (SELF) hit = hit && k[t] != i;
         if (hit && (k[t] < nMe || rr < rrC)) {
@@ -771,24 +775,36 @@ __device__ __forceinline__ void nlc_walk(const DevParams &p, int i, int c, const
         // indices fit 32 bits (ncell * cellCap < 2^31 is checked at set-up)
         const int o0 = C0 * cap, o1 = C1 * cap - n0, o2 = C2 * cap - n01;
         const bool self = q == 0 || wrapSelf;
-        for (int base = 0; base < tot; base += 4) {
-            float4 P[4];
-            float shz[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int v = min(base + t, tot - 1);
-                const bool a0 = v < n0, a1 = v < n01;
-                const int o = a0 ? o0 : (a1 ? o1 : o2);
-                shz[t] = a0 ? sz0 : (a1 ? sz1 : sz2);
-                P[t] = cpos[(unsigned)(o + v)];
+        // no lane of the wave sees this column through a periodic face: the shifts are all zero
+        const bool plain = __all(shx == 0.f && shy == 0.f && sz0 == 0.f && sz1 == 0.f && sz2 == 0.f);
+#define SGK_NLC_COLUMN(SELF, SHIFT)                                                                          \
+    for (int base = 0; base < tot; base += 4) {                                                             \
+        float4 P[4];                                                                                        \
+        float shz[4];                                                                                       \
+        _Pragma("unroll") for (int t = 0; t < 4; ++t) {                                                     \
+            const int v = min(base + t, tot - 1);                                                           \
+            const bool a0 = v < n0, a1 = v < n01;                                                           \
+            const int o = a0 ? o0 : (a1 ? o1 : o2);                                                         \
+            shz[t] = a0 ? sz0 : (a1 ? sz1 : sz2);                                                           \
+            P[t] = cpos[(unsigned)(o + v)];                                                                 \
+        }                                                                                                   \
+        nlc_chunk<SELF, KEEP, SHIFT>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl, \
+                                     fullNBL);                                                              \
+    }
+        if (self) {
+            if (plain) {
+                SGK_NLC_COLUMN(true, false);
+            } else {
+                SGK_NLC_COLUMN(true, true);
             }
-            if (self)
-                nlc_chunk<true, KEEP>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl,
-                                      fullNBL);
-            else
-                nlc_chunk<false, KEEP>(p, i, ic, A, shx, shy, shz, P, tot - base, nn, nljv, nclv, err, lj, cl,
-                                       fullNBL);
+        } else {
+            if (plain) {
+                SGK_NLC_COLUMN(false, false);
+            } else {
+                SGK_NLC_COLUMN(false, true);
+            }
         }
+#undef SGK_NLC_COLUMN
     }
     if (nn > p.nlCap) err |= SGK_FLAG_NL_OVERFLOW;
     nl_finish(p, i, false, ic, nn, nljv, nclv, err, nullptr, nlj, ncl, fullNN, st);
