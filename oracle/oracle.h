@@ -14,8 +14,11 @@
  * tangent.cpp compiles the same source a third time with forward-mode dual numbers (dual.hpp): the
  * FP64 values plus directional derivatives, the reference gradient of the adjoint.
  *
- * Pinning (see tests/test_oracle_*.py): EAM by fcc shell sums (closed form), the zero-force ideal
- * graphene sheet, two-atom LJ, central differences of the conservative EAM+LJ energy, brute-force
+ * Pinning (tests/test_oracle_*.py, tests/test_graphene_pinning.py): EAM by fcc shell sums (closed
+ * form), two-atom LJ, central differences of the conservative EAM+LJ energy, the graphene law by the
+ * zero-force ideal sheet, first-order in-plane and out-of-plane responses, the uniformly dilated
+ * sheet and, under finite deformation, each carbon's force against central differences of its own
+ * local energy (with a mutation test that plants mistakes in copies of this source), brute-force
  * O(N^2) neighbour lists and RDF, the Berendsen factor, leapfrog on a harmonic dimer, rigid
  * translation for the diffusion buffers, and the published bulk-Au known answer.
  */

@@ -114,6 +114,28 @@ def test_neighbour_list_bit_exact_wrapped_grid(gpu, prec, scheme, dims, grid):
     check_lists(p, prec, scheme, s.r)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("ordered", [False, True])
+@pytest.mark.parametrize("shape,grid,n_metal", [((4, 4), (6, 6, 2), 0), ((2, 2), (3, 3, 2), 0),
+                                                ((3, 2), (5, 3, 2), 0)])
+def test_neighbour_list_bit_exact_graphene_short_z_grid(gpu, prec, ordered, shape, grid, n_metal):
+    """Graphene in a box two cells tall: the sheet lies on the z cell boundary, so its atoms fall in
+    both z layers, and the z offsets -1 and +1 of every column reach the same cell, once directly and
+    once through the periodic face with a z shift.  This drives the carbon column walk through
+    repeated cells within one column and nonzero per-candidate z shifts; the (2, 2) sheet also has
+    only three cells along x and y.  Every atom is displaced at random so the sheet straddles the
+    boundary in both directions."""
+    s = sg.make_system(regime=2, ux=shape[0], uy=shape[1], n_metal=n_metal, temperature=300.0)
+    p = s.params
+    assert (p.cells.x, p.cells.y, p.cells.z) == grid
+    rng = np.random.default_rng(5)
+    r = s.r.copy()
+    r[:, :3] += rng.uniform(-0.08, 0.08, (len(r), 3)).astype(np.float32)
+    assert (r[:, 2] < 0).any() and (r[:, 2] >= 0).any()
+    print(f"[parity] NL graphene short-z grid {grid} {prec} ordered={ordered}", flush=True)
+    check_lists(p, prec, 3, r, ordered=ordered)
+
+
 @pytest.mark.parametrize("name", ["icosa", "medium"])
 def test_cell_ordered_lists_after_run(gpu, name):
     """nl_scheme 3 after a run: the carbon lists are built in the cell order made at the start of the
