@@ -152,6 +152,16 @@ const char *InitCoordsW(sg_float4 *dr, sg_float4 *hr, SimParams *hparams);
 char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hparams,
                       FILE *fResults, char *szPdbPath);
 
+/* ---- Device queries the reference driver makes itself (SurfaceGrowth.cpp:108-122, 315-318) ---- */
+
+/* Number of visible gfx950 devices; replaces the driver's cudaGetDeviceCount (SurfaceGrowth.cpp:109).
+ * Returns 0 when there is none or the runtime fails. */
+int sgDeviceCount(void);
+/* Largest cell grid (cells.x * cells.y * cells.z) the engine accepts with the reference's 64-atom
+ * cells; replaces the driver's g_hDeviceProp.maxGridSize[0] test (SurfaceGrowth.cpp:317), which
+ * limited the reference to one CUDA block per cell.  Here the limit is cells x 64 < 2^31. */
+unsigned int sgMaxCellGrid(void);
+
 /* ---- Host-side helpers of the reference driver (SurfaceGrowth.cpp), exported so a host written
  *      against this library can reuse them; behaviour follows the cited lines. ------------------ */
 
@@ -250,6 +260,11 @@ int sgStructSizes(int *out, int max);
 
 #ifdef __cplusplus
 }
+
+/* AccumProps (SurfaceGrowth.cu:2659-2686): zero (icode 0), accumulate (1) or average (2) the Prop
+ * fields of *hparams.  The reference defines it in its .cu with C++ linkage and its host driver
+ * calls it (SurfaceGrowth.cpp:417; prototype SurfaceGrowthProto.h:25), so it keeps that linkage. */
+void AccumProps(int icode, SimParams *hparams);
 #endif
 
 #endif /* SURFACEGROWTH_MI355X_H */

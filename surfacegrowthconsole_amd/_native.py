@@ -151,7 +151,8 @@ EXPORTED_SYMBOLS = [
     "sgDefaultOptions", "sgSetOptions", "sgGetOptions", "sgDefaultInput", "sgReadInputFile",
     "sgSetParams", "sgInitRand", "sgRandR", "sgVRand", "sgInitVels", "sgOpenResultFile",
     "sgWriteBackup", "sgReadBackup", "sgVersion", "sgStructSizes",
-    "sgDiffusionCreate", "sgDiffusionEval", "sgDiffusionDestroy",
+    "sgDiffusionCreate", "sgDiffusionEval", "sgDiffusionDestroy", "sgDeviceCount", "sgMaxCellGrid",
+    "AccumProps",
     "sgk_default_config", "sgk_create", "sgk_destroy", "sgk_last_error", "sgk_device_count",
     "sgk_upload", "sgk_download", "sgk_upload_f64", "sgk_download_f64", "sgk_set_run_state",
     "sgk_bin", "sgk_get_cells", "sgk_build_nebr", "sgk_build_nl_order", "sgk_get_nebr", "sgk_get_carbon_lists",

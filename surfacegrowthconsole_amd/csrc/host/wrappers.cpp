@@ -46,6 +46,13 @@ void CudaInitW(int argc, char **argv) {
         std::printf("Detected %d HIP capable device(s)\n", n);
 }
 
+int sgDeviceCount(void) {
+    const int n = sgk_device_count();
+    return n > 0 ? n : 0;
+}
+
+unsigned int sgMaxCellGrid(void) { return (unsigned int)((2147483647u) / SG_BLOCK_SIZE); }
+
 void SetParametersW(SimParams *hostParams) {
     if (!hostParams) {
         std::printf("Memory error while copying parameters\n");
@@ -206,3 +213,9 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
 }
 
 }  // extern "C"
+
+// C++ linkage, as the reference's definition (SurfaceGrowth.cu:2659) and its driver's call
+// (SurfaceGrowth.cpp:417) have it.
+void AccumProps(int icode, SimParams *hparams) {
+    if (hparams) sg::accum_props(icode, *hparams);
+}
