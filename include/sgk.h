@@ -94,6 +94,10 @@ typedef struct sgk_record {
     double shearCount;    /* atoms sheared in this step */
     int nMolDeposited;    /* metal atoms deposited so far (regime 1) */
     int pad;
+    double virial;        /* pair virial 1/2 sum_i sum_k f_ik |d_ik|^2 of the step's EAM and
+                             metal-carbon LJ forces (every LJ term the carbon side counts; not the
+                             graphene law, which has no pair reaction); W = -3V dE/dV for those
+                             terms.  No counterpart in the reference.  0 from sgk_eval_props. */
 } sgk_record;
 
 void sgk_default_config(sgk_config *cfg);
@@ -154,6 +158,17 @@ int sgk_eval_props(sgk_ctx *ctx, sgk_record *rec);
 int sgk_rdf_clear(sgk_ctx *ctx);
 int sgk_rdf_accumulate(sgk_ctx *ctx);
 int sgk_rdf_get(sgk_ctx *ctx, unsigned int *hist);
+/* Centre-of-mass diffusion accumulated on the device, in FP64 (EvalDiffusion, AccumDiffusion,
+ * PrintDiffusion: SurfaceGrowth.cu:2865-2952).  Once enabled, every step that the reference samples
+ * (regime 2, step > stepEquil, (step - stepEquil) % stepDiffuse == 0) of sgk_run adds one sample of
+ * the metal centre of mass to nBuffDiffuse staggered buffers of nValDiffuse values; every
+ * limitDiffuseAv completed buffers give one table of D_j (j = 0..nValDiffuse-1, reduced units, the
+ * reference's normalisation).  Sizes come from the SimParams diffusion fields; the last
+ * `maxTables` tables are kept.  Enabling again restarts the buffers. */
+int sgk_diffusion_enable(sgk_ctx *ctx, int maxTables);
+/* Completed tables so far in *nTables; the most recent min(nTables, maxTables) are copied to
+ * tables[k * nValDiffuse + j], oldest first (tables may be NULL). */
+int sgk_diffusion_get(sgk_ctx *ctx, double *tables, int maxTables, int *nTables);
 
 /* --- device-resident loop --------------------------------------------------------------------- */
 /* Advance nsteps steps in the order of DoComputationsW (SurfaceGrowth.cu:2141-2489) without host
@@ -162,6 +177,10 @@ int sgk_rdf_get(sgk_ctx *ctx, unsigned int *hist);
  * RDF accumulation every rdfEvery steps (0: none).  records[nsteps] receives one sgk_record per
  * step (may be NULL).  nsteps must not exceed records_capacity. */
 int sgk_run(sgk_ctx *ctx, int nsteps, int rdfEvery, sgk_record *records);
+/* Capture and instantiate, without running anything, the step graph that the next
+ * sgk_run(ctx, nsteps, rdfEvery, ...) from the current step will launch, so that call does not pay
+ * for the capture.  A no-op without graphs.  No counterpart in the reference. */
+int sgk_prepare_run(sgk_ctx *ctx, int nsteps, int rdfEvery);
 /* Milliseconds of device time spent in the last sgk_run (HIP events on the engine stream). */
 double sgk_last_run_ms(sgk_ctx *ctx);
 

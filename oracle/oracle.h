@@ -97,6 +97,11 @@ void orc_forces_f32(const orc_params *p, float *pos4, const int *NN, const int *
                     const float *rho, float *acc, float *fric);
 void orc_forces_f64(const orc_params *p, double *pos4, const int *NN, const int *NBL,
                     const double *rho, double *acc, double *fric);
+/* orc_forces plus each atom's pair virial 1/2 sum_k f_ik |d_ik|^2 of its EAM and LJ terms. */
+void orc_forces_virial_f32(const orc_params *p, float *pos4, const int *NN, const int *NBL, const float *rho,
+                           float *acc, float *fric, float *vir);
+void orc_forces_virial_f64(const orc_params *p, double *pos4, const int *NN, const int *NBL, const double *rho,
+                           double *acc, double *fric, double *vir);
 /* Embedding function F(rho) and F'(rho) (.cu:105-118, 197-210). */
 float orc_embed_f32(const orc_params *p, float rho, float *dF);
 double orc_embed_f64(const orc_params *p, double rho, double *dF);

@@ -70,6 +70,7 @@ def lib():
                 "orc_nebr": (C.c_int, [vp, vp, vp, vp, vp, vp]),
                 "orc_rho": (None, [vp, vp, vp, vp, vp]),
                 "orc_forces": (None, [vp, vp, vp, vp, vp, vp, vp]),
+                "orc_forces_virial": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
                 "orc_embed": (R, [vp, R, vp]),
                 "orc_pair": (None, [vp, R, vp]),
                 "orc_leapfrog": (None, [vp, C.c_int, vp, vp, vp]),
@@ -176,6 +177,18 @@ class Oracle:
         fr = np.zeros(max(self.n_metal, 1), self.dt)
         self.f("orc_forces")(C.byref(self.p), _p(r), _p(nn), _p(nbl), _p(rho), _p(acc), _p(fr))
         return acc, r[:, 3].copy(), fr[: self.n_metal]
+
+    def virial(self, r, nn, nbl, rho=None):
+        """Per-atom pair virial 1/2 sum_k f_ik |d_ik|^2 of the EAM and LJ terms (N,)."""
+        r = self.arr(r).copy()
+        if rho is None:
+            rho = self.rho(r, nn, nbl)
+        rho = np.ascontiguousarray(np.resize(rho, max(self.n_metal, 1)), self.dt)
+        acc = np.zeros((self.n, 3), self.dt)
+        fr = np.zeros(max(self.n_metal, 1), self.dt)
+        vir = np.zeros(self.n, self.dt)
+        self.f("orc_forces_virial")(C.byref(self.p), _p(r), _p(nn), _p(nbl), _p(rho), _p(acc), _p(fr), _p(vir))
+        return vir
 
     def embed(self, rho):
         d = (C.c_float if self.sfx == "f32" else C.c_double)()

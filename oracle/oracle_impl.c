@@ -262,16 +262,19 @@ static inline REAL lj_pair(const orc_params *p, REAL rr, REAL *u) {
     return K(48.) * PAR_EPS(p) * rri3 * (rri3 - K(0.5)) * rri / s2;
 }
 
-/* ComputeForcesK (.cu:776-1091), one atom at a time, neighbours in list order. */
-void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *NBL, const REAL *rho,
-                    REAL *acc, REAL *fric) {
+/* ComputeForcesK (.cu:776-1091), one atom at a time, neighbours in list order.  vir (may be NULL)
+ * receives each atom's pair virial 1/2 sum_k f_ik |d_ik|^2 over the pair-decomposable forces it
+ * evaluates - EAM and metal-carbon LJ, every LJ visit of the carbon loop counted - not the graphene
+ * law, whose terms have no pair reaction (no counterpart in the reference). */
+static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, const int *NBL, const REAL *rho,
+                            REAL *acc, REAL *fric, REAL *vir) {
     const int nMol = p->nMol, nMe = p->nMolMe;
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
     const REAL z0 = p->z_0, rrC = p->rrCutC, mu_r = p->mu_r, mu_p = p->mu_p, mu_t = p->mu_theta,
                th0 = p->theta_0;
     for (int i = 0; i < nMol; ++i) {
         const int metal = i < nMe;
-        REAL fx = 0, fy = 0, fz = 0, frx = 0, uSum = 0;
+        REAL fx = 0, fy = 0, fz = 0, frx = 0, uSum = 0, wSum = 0;
         const REAL Ax = pos4[4 * i], Ay = pos4[4 * i + 1], Az = pos4[4 * i + 2];
         const int nn = NN[i];
         const REAL rhoi = metal ? rho[i] : 0;
@@ -294,6 +297,7 @@ void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *N
                     fx += fc * dx;
                     fy += fc * dy;
                     fz += fc * dz;
+                    wSum += fc * rr;
                     if (rr < (REAL)p->rrCutEam) uSum += u;
                 } else { /* LJ with carbon (.cu:878-896) */
                     REAL u, fc = lj_pair(p, rr, &u);
@@ -303,6 +307,7 @@ void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *N
                     fy += fc * dy;
                     fz += fc * dz;
                     frx += fc * dx;
+                    wSum += fc * rr;
                 }
                 continue;
             }
@@ -313,6 +318,7 @@ void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *N
                 fx += fc * dx;
                 fy += fc * dy;
                 fz += fc * dz;
+                wSum += fc * rr;
                 continue;
             }
             /* carbon-carbon: first three neighbours within rCutC are the bonds (.cu:921-984) */
@@ -401,7 +407,18 @@ void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *N
             acc[3 * i + 2] = fz;
             pos4[4 * i + 3] = K(0.5) * uSum;
         }
+        if (vir) vir[i] = K(0.5) * wSum;
     }
+}
+
+void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *NBL, const REAL *rho,
+                    REAL *acc, REAL *fric) {
+    FN(forces_impl)(p, pos4, NN, NBL, rho, acc, fric, NULL);
+}
+
+void FN(orc_forces_virial)(const orc_params *p, REAL *pos4, const int *NN, const int *NBL, const REAL *rho,
+                           REAL *acc, REAL *fric, REAL *vir) {
+    FN(forces_impl)(p, pos4, NN, NBL, rho, acc, fric, vir);
 }
 
 /* ------------------------------------------------------------------------------------------ */

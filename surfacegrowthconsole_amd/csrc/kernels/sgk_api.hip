@@ -211,6 +211,61 @@ void sgk::make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &
     p.nlScheme = cfg.nl_scheme;
 }
 
+int sgk::prepare_ctx(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx *c, std::string &err) {
+    sgk_config cfg;
+    if (cfgIn)
+        cfg = *cfgIn;
+    else
+        sgk_default_config(&cfg);
+    sgk_config def;
+    sgk_default_config(&def);
+    if (cfg.nl_capacity <= 0) cfg.nl_capacity = sp->iNebrMax > 0 ? sp->iNebrMax : 128;
+    if (cfg.cell_capacity <= 0) cfg.cell_capacity = def.cell_capacity;
+    if (cfg.lj_capacity <= 0) cfg.lj_capacity = def.lj_capacity;
+    if (cfg.close_capacity <= 0) cfg.close_capacity = def.close_capacity;
+    if (cfg.close_margin <= 0.f) cfg.close_margin = def.close_margin;
+    if (cfg.block_size <= 0) cfg.block_size = def.block_size;
+    if (cfg.records_capacity <= 0) cfg.records_capacity = def.records_capacity;
+    auto bad = [&err](int code, const char *msg) {
+        err = msg;
+        return code;
+    };
+    if (sp->nMol <= 0 || sp->nMolMe < 0 || sp->nMolMe > sp->nMol) return bad(SGK_E_ARG, "invalid atom counts");
+    if (sp->nMol >= (1 << 24)) return bad(SGK_E_UNSUPPORTED, "more than 2^24 atoms");
+    if (cfg.nl_capacity > 255) return bad(SGK_E_UNSUPPORTED, "neighbour capacity above 255");
+    if (cfg.cell_capacity > 64 || cfg.cell_capacity % 8)
+        return bad(SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64 (one wave per cell)");
+    if (cfg.close_capacity % 8) return bad(SGK_E_UNSUPPORTED, "close capacity must be a multiple of 8");
+    if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return bad(SGK_E_ARG, "empty cell grid");
+    if ((long long)sp->cells.x * sp->cells.y * sp->cells.z * cfg.cell_capacity >= (1ll << 31))
+        return bad(SGK_E_UNSUPPORTED, "cell grid too large: cells x cell capacity must stay below 2^31");
+    if (sp->iRegime < 0 || sp->iRegime > 2) return bad(SGK_E_ARG, "regime must be 0, 1 or 2");
+    if (sp->sizeHistRdf <= 0 || sp->sizeHistRdf > 8192) return bad(SGK_E_ARG, "RDF bins must be 1..8192");
+    c->sp = *sp;
+    c->cfg = cfg;
+    c->mixed = cfg.precision == SG_PREC_MIXED;
+    sgk::make_dev_params(c->sp, c->cfg, c->dp);
+    const DevParams &p = c->dp;
+    // lanes per metal atom: enough waves to cover the machine for small clusters
+    int G = cfg.metal_lanes;
+    // measured on MI355X (sgk_autotune): 4 lanes for 100k metal atoms, 8 for 4k, 16 for small clusters
+    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 4 : p.nMolMe >= 2048 ? 8 : 16;
+    c->metalLanes = G;
+    c->block = 256;
+    // metal ranges padded to a multiple of 8 workgroups so the carbon range starts on XCD 0
+    c->nbM = (int)((((long long)p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+    c->nbC = (p.nC + 255) / 256;
+    c->nbRho = c->nbM;
+    c->nbProps = (p.nMol + 255) / 256;
+    // partials for the largest force grid any launch shape gives (16 lanes per metal atom), then one
+    // chunk total per 64 partials (two-stage step reduction)
+    const size_t nbMmax = (size_t)((((long long)p.nMolMe * 16 + 255) / 256 + 7) / 8 * 8);
+    const size_t nParts = std::max<size_t>(std::max<size_t>(nbMmax, (size_t)c->nbM) + (size_t)c->nbC,
+                                           (size_t)c->nbProps);
+    c->part2Off = (int)nParts;
+    return SGK_OK;
+}
+
 extern "C" {
 
 void sgk_default_config(sgk_config *cfg) {
@@ -241,39 +296,18 @@ const char *sgk_last_error(const sgk_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     if (!sp || !out) return fail(nullptr, SGK_E_ARG, "null argument");
     *out = nullptr;
-    sgk_config cfg;
-    if (cfgIn)
-        cfg = *cfgIn;
-    else
-        sgk_default_config(&cfg);
-    sgk_config def;
-    sgk_default_config(&def);
-    if (cfg.nl_capacity <= 0) cfg.nl_capacity = sp->iNebrMax > 0 ? sp->iNebrMax : 128;
-    if (cfg.cell_capacity <= 0) cfg.cell_capacity = def.cell_capacity;
-    if (cfg.lj_capacity <= 0) cfg.lj_capacity = def.lj_capacity;
-    if (cfg.close_capacity <= 0) cfg.close_capacity = def.close_capacity;
-    if (cfg.close_margin <= 0.f) cfg.close_margin = def.close_margin;
-    if (cfg.block_size <= 0) cfg.block_size = def.block_size;
-    if (cfg.records_capacity <= 0) cfg.records_capacity = def.records_capacity;
-    if (sp->nMol <= 0 || sp->nMolMe < 0 || sp->nMolMe > sp->nMol)
-        return fail(nullptr, SGK_E_ARG, "invalid atom counts");
-    if (sp->nMol >= (1 << 24)) return fail(nullptr, SGK_E_UNSUPPORTED, "more than 2^24 atoms");
-    if (cfg.nl_capacity > 255) return fail(nullptr, SGK_E_UNSUPPORTED, "neighbour capacity above 255");
-    if (cfg.cell_capacity > 64 || cfg.cell_capacity % 8)
-        return fail(nullptr, SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64 (one wave per cell)");
-    if (cfg.close_capacity % 8) return fail(nullptr, SGK_E_UNSUPPORTED, "close capacity must be a multiple of 8");
-    if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return fail(nullptr, SGK_E_ARG, "empty cell grid");
-    if ((long long)sp->cells.x * sp->cells.y * sp->cells.z * cfg.cell_capacity >= (1ll << 31))
-        return fail(nullptr, SGK_E_UNSUPPORTED, "cell grid too large: cells x cell capacity must stay below 2^31");
-    if (sp->iRegime < 0 || sp->iRegime > 2) return fail(nullptr, SGK_E_ARG, "regime must be 0, 1 or 2");
-    if (sp->sizeHistRdf <= 0 || sp->sizeHistRdf > 8192) return fail(nullptr, SGK_E_ARG, "RDF bins must be 1..8192");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, SGK_E_HIP, "no HIP device");
-
     sgk_ctx *c = new sgk_ctx();
-    c->sp = *sp;
-    c->cfg = cfg;
-    c->mixed = cfg.precision == SG_PREC_MIXED;
+    {
+        std::string msg;
+        const int rc0 = sgk::prepare_ctx(sp, cfgIn, c, msg);
+        if (rc0 != SGK_OK) {
+            delete c;
+            return fail(nullptr, rc0, msg);
+        }
+    }
+    const sgk_config &cfg = c->cfg;
     (void)hipGetDevice(&c->device);
     int rc;
 #define SGK_TRY(x)                   \
@@ -292,19 +326,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
             return SGK_E_HIP;
         }
     }
-    sgk::make_dev_params(c->sp, c->cfg, c->dp);
     const DevParams &p = c->dp;
-    // lanes per metal atom: enough waves to cover the machine for small clusters
-    int G = cfg.metal_lanes;
-    // measured on MI355X (sgk_autotune): 4 lanes for 100k metal atoms, 8 for 4k, 16 for small clusters
-    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 4 : p.nMolMe >= 2048 ? 8 : 16;
-    c->metalLanes = G;
-    c->block = 256;
-    // metal ranges padded to a multiple of 8 workgroups so the carbon range starts on XCD 0
-    c->nbM = (int)((((long long)p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
-    c->nbC = (p.nC + 255) / 256;
-    c->nbRho = c->nbM;
-    c->nbProps = (p.nMol + 255) / 256;
     const size_t R = c->mixed ? 8 : 4;
     const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
     SGK_TRY(dalloc(c, (char **)&c->pos, nMol * (c->mixed ? 32 : 16), true));
@@ -345,11 +367,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
         SGK_TRY(dalloc(c, &c->fullNBL, nMol * (size_t)p.nlCap, false));
     }
-    // partials for the largest force grid any launch shape gives (16 lanes per metal atom)
-    const size_t nbMmax = (size_t)((((long long)p.nMolMe * 16 + 255) / 256 + 7) / 8 * 8);
-    const size_t nParts = std::max<size_t>(std::max<size_t>(nbMmax, (size_t)c->nbM) + (size_t)c->nbC,
-                                           (size_t)c->nbProps);
-    c->part2Off = (int)nParts;  // then one chunk total per 64 partials (two-stage step reduction)
+    const size_t nParts = (size_t)c->part2Off;
     SGK_TRY(dalloc(c, (char **)&c->partials, (nParts + nParts / 64 + 2) * sgk::partial_bytes(c), true));
     SGK_TRY(dalloc(c, &c->state, 1, true));
     SGK_TRY(dalloc(c, &c->records, (size_t)cfg.records_capacity, true));
@@ -392,6 +410,7 @@ int sgk_destroy(sgk_ctx *c) {
                     c->nlOrder, c->ordCnt, c->ordKey, c->ordSlot, c->ordBlock};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (c->diffMem) (void)hipFree(c->diffMem);
     if (c->hostState) (void)hipHostFree(c->hostState);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -757,6 +776,68 @@ int sgk_rdf_accumulate(sgk_ctx *c) {
     return SGK_OK;
 }
 
+int sgk_diffusion_enable(sgk_ctx *c, int maxTables) {
+    if (!c) return SGK_E_ARG;
+    const SimParams &s = c->sp;
+    if (maxTables <= 0) return fail(c, SGK_E_ARG, "maxTables must be positive");
+    if (s.nValDiffuse <= 0 || s.nBuffDiffuse <= 0 || s.stepDiffuse <= 0 || s.limitDiffuseAv <= 0)
+        return fail(c, SGK_E_ARG, "diffusion sizes (nValDiffuse, nBuffDiffuse, stepDiffuse, limitDiffuseAv) must be positive");
+    if (s.nBuffDiffuse > sgk::kMaxDiffuseBuffers) return fail(c, SGK_E_UNSUPPORTED, "more than 256 diffusion buffers");
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->diffMem) {
+        SGK_HIP(c, hipFree(c->diffMem));
+        c->diffMem = nullptr;
+        c->diff = sgk::DiffusionArgs{};
+    }
+    const size_t nb = s.nBuffDiffuse, nv = s.nValDiffuse;
+    const size_t ints = (nb + 2 + 1) / 2 * 2;  // counts and the two state words, padded to 8 bytes
+    const size_t doubles = 6 * nb + nb * nv + nv + (size_t)maxTables * nv;
+    const size_t bytes = 4 * ints + 8 * doubles;
+    SGK_HIP(c, hipMalloc(&c->diffMem, bytes));
+    c->bytes += (long long)bytes;
+    sgk::DiffusionArgs &a = c->diff;
+    a.count = (int *)c->diffMem;
+    a.state = a.count + nb;
+    double *d = (double *)((char *)c->diffMem + 4 * ints);
+    a.org = d;
+    a.tru = d + 3 * nb;
+    a.rr = d + 6 * nb;
+    a.av = a.rr + nb * nv;
+    a.out = a.av + nv;
+    a.nVal = (int)nv;
+    a.nBuff = (int)nb;
+    a.stepDiffuse = s.stepDiffuse;
+    a.limitAv = s.limitDiffuseAv;
+    a.cap = maxTables;
+    a.deltaT = (double)s.deltaT;
+    a.L[0] = (double)s.region.x;
+    a.L[1] = (double)s.region.y;
+    a.L[2] = (double)s.region.z;
+    sgk::launch_diffuse_reset(c);
+    sgk::drop_graphs(c);  // captured steps carry the sampling launches (or their absence)
+    SGK_HIP(c, hipGetLastError());
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    return SGK_OK;
+}
+
+int sgk_diffusion_get(sgk_ctx *c, double *tables, int maxTables, int *nTables) {
+    if (!c) return SGK_E_ARG;
+    if (!c->diff.count) return fail(c, SGK_E_STATE, "diffusion is not enabled: call sgk_diffusion_enable");
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    int st[2] = {0, 0};
+    SGK_HIP(c, hipMemcpy(st, c->diff.state, sizeof(st), hipMemcpyDeviceToHost));
+    if (nTables) *nTables = st[1];
+    if (tables && maxTables > 0) {
+        const int keep = std::min(std::min(st[1], maxTables), c->diff.cap);
+        const size_t nv = (size_t)c->diff.nVal;
+        for (int k = 0; k < keep; ++k) {
+            const int idx = (st[1] - keep + k) % c->diff.cap;
+            SGK_HIP(c, hipMemcpy(tables + (size_t)k * nv, c->diff.out + (size_t)idx * nv, 8 * nv, hipMemcpyDeviceToHost));
+        }
+    }
+    return SGK_OK;
+}
+
 int sgk_rdf_get(sgk_ctx *c, unsigned int *hist) {
     if (!c || !hist) return SGK_E_ARG;
     SGK_HIP(c, hipStreamSynchronize(c->stream));
@@ -780,7 +861,10 @@ std::string sgk::step_flag_string(sgk_ctx *c, int step0, int nsteps, int rdfEver
         // with deltaF = 0 the shear stays 0 and ApplyShearK adds nothing: no launch needed
         const bool shear = s.iRegime == 2 && s.nMolMe > 0 && step > s.stepEquil + s.stepCool && s.deltaF != 0.f;
         const bool rdf = rdfEvery > 0 && step % rdfEvery == 0;
-        b = (thermo ? 1 : 0) | (shear ? 2 : 0) | (rdf ? 4 : 0) | (observe ? 8 : 0);
+        // diffusion sample (SurfaceGrowth.cu:2315-2322), when the device buffers are enabled
+        const bool msd = c->diff.count && s.iRegime == 2 && s.nMolMe > 0 && step > s.stepEquil &&
+                         (step - s.stepEquil) % s.stepDiffuse == 0;
+        b = (thermo ? 1 : 0) | (shear ? 2 : 0) | (rdf ? 4 : 0) | (observe ? 8 : 0) | (msd ? 16 : 0);
         f[k] = (char)('a' + b);
     }
     return f;
@@ -805,6 +889,7 @@ static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
         if (b & 1) sgk::launch_thermostat(c, 0, 0.0, 0);
         if (b & 2) sgk::launch_shear(c);
         if (b & 4) sgk::launch_rdf(c);
+        if (b & 16) sgk::launch_diffuse(c);
         if (b & 8) sgk::launch_adh_obs(c, c->adj->d.part, c->adj->d.adh, 1);
     }
 }
@@ -820,6 +905,50 @@ int sgk_run(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records) {
 
 }  // extern "C"
 
+// The instantiated graph of a step pattern: from the cache, or captured now.  The cache holds at most
+// 8 patterns per context; the least used is destroyed first (the stream has been synchronised
+// since its last launch, by the run that launched it).
+static int step_graph(sgk_ctx *c, const std::string &flags, hipGraphExec_t *out) {
+    auto it = c->graphs.find(flags);
+    if (it == c->graphs.end()) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        SGK_HIP(c, hipStreamIsCapturing(c->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) return fail(c, SGK_E_STATE, "the engine stream is already being captured");
+        if (c->graphs.size() >= 8) {
+            auto victim = c->graphs.begin();
+            for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
+                if (g->second.uses < victim->second.uses) victim = g;
+            SGK_HIP(c, hipGraphExecDestroy(victim->second.exec));
+            c->graphs.erase(victim);
+        }
+        hipGraph_t graph = nullptr;
+        SGK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
+        enqueue_steps(c, flags);
+        g_capturing = false;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        sgk::GraphEntry ge;
+        e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+        it = c->graphs.emplace(flags, ge).first;
+    }
+    it->second.uses += 1;
+    *out = it->second.exec;
+    return SGK_OK;
+}
+
+extern "C" int sgk_prepare_run(sgk_ctx *c, int nsteps, int rdfEvery) {
+    if (!c) return SGK_E_ARG;
+    if (nsteps <= 0 || nsteps > c->cfg.records_capacity) return fail(c, SGK_E_ARG, "nsteps must be in 1..records_capacity");
+    if (!c->cfg.use_graphs || c->profiling) return SGK_OK;
+    hipGraphExec_t exec = nullptr;
+    const int rc = step_graph(c, sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, false), &exec);
+    if (rc == SGK_OK) c->graphs[sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, false)].uses -= 1;
+    return rc;
+}
+
 int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, double *adh) {
     if (!c) return SGK_E_ARG;
     if (nsteps <= 0) return SGK_OK;
@@ -832,30 +961,10 @@ int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, do
     // only which lane builds which atom's lists, so it is refreshed now and then, outside the graph
     if (c->nlOrder && (!c->orderValid || c->orderAge >= sgk::kNlOrderSteps)) sgk::build_nl_order(c);
     if (c->cfg.use_graphs && !c->profiling) {
-        auto it = c->graphs.find(flags);
-        if (it == c->graphs.end()) {
-            if (c->graphs.size() >= 8) {  // bounded cache: drop the least used
-                auto victim = c->graphs.begin();
-                for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
-                    if (g->second.uses < victim->second.uses) victim = g;
-                (void)hipGraphExecDestroy(victim->second.exec);
-                c->graphs.erase(victim);
-            }
-            hipGraph_t graph = nullptr;
-            SGK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            g_capturing = true;
-            enqueue_steps(c, flags);
-            g_capturing = false;
-            hipError_t e = hipStreamEndCapture(c->stream, &graph);
-            if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-            sgk::GraphEntry ge;
-            e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
-            it = c->graphs.emplace(flags, ge).first;
-        }
-        it->second.uses += 1;
-        SGK_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
+        hipGraphExec_t exec = nullptr;
+        const int rc = step_graph(c, flags, &exec);
+        if (rc != SGK_OK) return rc;
+        SGK_HIP(c, hipGraphLaunch(exec, c->stream));
     } else {
         enqueue_steps(c, flags);
     }

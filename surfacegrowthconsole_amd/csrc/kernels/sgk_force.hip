@@ -211,7 +211,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
     const int gt = blk * blockDim.x + threadIdx.x;
     const int i = gt / G, l = gt % G;
     const bool active = i < p.nMolMe;
-    T fx = T(0), fy = T(0), fz = T(0), frx = T(0), u = T(0);
+    T fx = T(0), fy = T(0), fz = T(0), frx = T(0), u = T(0), w = T(0);
     P4 Ai;
     if (active) {
         Ai = pos[i];
@@ -251,6 +251,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
                         fx += fc * dx;
                         fy += fc * dy;
                         fz += fc * dz;
+                        w += fc * rr;
                         u += phi;
                     }
                 } else if (rrr < (R)p.rrCutLJ) {
@@ -260,6 +261,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
                     fy += fc * dy;
                     fz += fc * dz;
                     frx += fc * dx;
+                    w += fc * rr;
                     u += ul;
                 }
             }
@@ -270,6 +272,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
     fz = group_sum<G>(fz);
     frx = group_sum<G>(frx);
     u = group_sum<G>(u);
+    w = group_sum<G>(w);
     Partial<A> q;
     partial_identity(q);
     if (active && l == 0) {
@@ -296,6 +299,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
         }
         const A m = (A)p.massMe;
         q.u = (A)e;
+        q.vir = (A)(T(0.5f) * w);
         q.fr = (A)frx;
         q.com[0] = (A)Ai.x;
         q.com[1] = (A)Ai.y;
@@ -488,7 +492,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     // (SurfaceGrowth.cu:989), so after the third bond (list position j0) the loop resumes at
     // position 4: entries at positions <= j0 count once, positions 4..j0 twice, and with j0 = 2
     // position 3 not at all.
-    T gx = (T)fx, gy = (T)fy, gz = (T)fz, gu = (T)u;
+    T gx = (T)fx, gy = (T)fy, gz = (T)fz, gu = (T)u, gw = T(0);
     const int nl = nlj[ic];
     for (int qq = 0; qq < nl; ++qq) {
         const unsigned e = lj[(size_t)qq * p.nC + ic];
@@ -505,6 +509,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
             gy += fc * (T)dyr;
             gz += fc * (T)dzr;
             gu += ul;
+            gw += fc * (T)rrr;
         }
     }
     const R ax = (R)gx, ay = (R)gy, az = (R)gz;  // carbon mass is the unit
@@ -528,6 +533,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         vel[2 * p.nMol + i] = integrate == 2 ? vz + hdt * az : vz;
     }
     q.u = (A)e;
+    q.vir = (A)(T(0.5f) * gw);
     q.vs[0] = (A)vx;
     q.vs[1] = (A)vy;
     q.vs[2] = (A)vz;

@@ -77,6 +77,8 @@ struct sgk_ctx {
     std::map<std::string, sgk::KernelStat> stats;
     std::string err;
     sgk::Adjoint *adj = nullptr;                        // adjoint storage (sgk_adjoint_*), on demand
+    sgk::DiffusionArgs diff = {};                       // device diffusion (sgk_diffusion_enable), on demand
+    void *diffMem = nullptr;
 };
 
 // A batch of replicas advanced together (sgk_batch_*): shared arrays, one range per replica.
@@ -155,6 +157,10 @@ namespace sgk {
 
 // Device parameters of one system (sgk_api.hip).
 void make_dev_params(const SimParams &s, const sgk_config &cfg, DevParams &p);
+// Validates the parameters and the configuration (defaults filled in) and sets c->sp, cfg, mixed, dp
+// and the launch geometry (lanes per metal atom, block counts, partials layout); allocates nothing.
+// Returns SGK_OK or an error code with the reason in err.
+int prepare_ctx(const SimParams *sp, const sgk_config *cfg, sgk_ctx *c, std::string &err);
 
 // sgk_nebr.hip
 void launch_bin(sgk_ctx *c, bool force);
@@ -188,11 +194,14 @@ void launch_finalize(sgk_ctx *c, int mode);          // 0 step, 1 evaluate only
 void launch_thermostat(sgk_ctx *c, int stepOverride, double vvOverride, int useOverride);
 void launch_shear(sgk_ctx *c);
 void launch_reset_records(sgk_ctx *c);
+void launch_diffuse(sgk_ctx *c);
+void launch_diffuse_reset(sgk_ctx *c);
 
 // sgk_api.hip: error reporting, device flags, the step-kernel flags of a run and the device loop.
 int set_error(sgk_ctx *c, int code, const std::string &msg);
 int check_device_flags(sgk_ctx *c);
-// One character per step, 'a' + bits: 1 thermostat, 2 shear, 4 RDF, 8 adhesion energy recorded.
+// One character per step, 'a' + bits: 1 thermostat, 2 shear, 4 RDF, 8 adhesion energy recorded,
+// 16 diffusion sample.
 std::string step_flag_string(sgk_ctx *c, int step0, int nsteps, int rdfEvery, bool observe);
 // sgk_run with, if adh is not null, the adhesion energy of every step copied to adh[nsteps].
 int run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, double *adh);

@@ -57,6 +57,7 @@ struct CellBufs {
 template <class T>
 struct Partial {
     T u, fr, com[3], cmv[3], vs[3], vv, vmax, bmin[3], bmax[3];
+    T vir;  // pair virial 1/2 sum f_ik |d_ik|^2 of the EAM and LJ terms (force pass only)
 };
 
 // Device-resident run state (one per context).
@@ -74,6 +75,21 @@ struct DevState {
     double cmVelX;         // metal centre-of-mass velocity x of the last step
     unsigned int shearCount;
     int nMolDeposited;     // metal atoms deposited so far (regime 1)
+};
+
+// Device buffers of the centre-of-mass diffusion (sgk_diffusion_enable): nBuff staggered buffers of
+// nVal squared displacements each, in FP64.
+constexpr int kMaxDiffuseBuffers = 256;
+struct DiffusionArgs {
+    int *count;        // [nBuff] samples taken per buffer (negative: not started)
+    int *state;        // [2] buffers averaged so far, tables completed
+    double *org, *tru; // [3 nBuff] origin and unwrapped position of each buffer
+    double *rr;        // [nBuff nVal]
+    double *av;        // [nVal] running average
+    double *out;       // [cap nVal] completed tables (ring)
+    int nVal, nBuff, stepDiffuse, limitAv, cap;
+    double deltaT;     // the float time step, widened
+    double L[3];       // the float box, widened
 };
 
 }  // namespace sgk

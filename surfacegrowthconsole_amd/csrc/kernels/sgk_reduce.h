@@ -9,7 +9,7 @@ namespace sgk {
 
 template <class T>
 __device__ __forceinline__ void partial_identity(Partial<T> &q) {
-    q.u = q.fr = q.vv = q.vmax = T(0);
+    q.u = q.fr = q.vv = q.vmax = q.vir = T(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         q.com[d] = q.cmv[d] = q.vs[d] = T(0);
@@ -21,6 +21,7 @@ __device__ __forceinline__ void partial_identity(Partial<T> &q) {
 template <class T>
 __device__ __forceinline__ void partial_combine(Partial<T> &r, const Partial<T> &s) {
     r.u += s.u;
+    r.vir += s.vir;
     r.fr += s.fr;
     r.vv += s.vv;
     r.vmax = s.vmax > r.vmax ? s.vmax : r.vmax;
@@ -35,10 +36,11 @@ __device__ __forceinline__ void partial_combine(Partial<T> &r, const Partial<T> 
 }
 
 // Wave reduction of a Partial (result in lane 63).  METAL = false reduces only the fields a
-// carbon atom contributes (energy, velocity sums, v^2 max); the others stay at their identity.
+// carbon atom contributes (energy, virial, velocity sums, v^2 max); the others stay at their identity.
 template <bool METAL, class T>
 __device__ __forceinline__ void partial_wave(Partial<T> &q) {
     q.u = wave_reduce63<OpSum>(q.u);
+    q.vir = wave_reduce63<OpSum>(q.vir);
     q.vv = wave_reduce63<OpSum>(q.vv);
     q.vmax = wave_reduce63<OpMax>(q.vmax);
 #pragma unroll
@@ -128,6 +130,7 @@ __device__ void finalize_body(const DevParams &p, const Partial<A> *part, int nP
     rec.vvSum = (double)vvSum;
     rec.vvMax = (double)P.vmax;
     rec.uSum = (double)P.u;
+    rec.virial = (double)P.vir;
     rec.frict = (p.iRegime != 0 && nMe) ? (double)P.fr : 0.0;
     if (mode == 1) {
         rec.step = st->step;

@@ -280,6 +280,94 @@ __global__ __launch_bounds__(256) void kb_shear(BatchArgs a) {
                   a.recCap);
 }
 
+// Centre-of-mass diffusion on the device (EvalDiffusion + AccumDiffusion, SurfaceGrowth.cu:2898-2952),
+// one workgroup, in FP64: after the step's reduction, the metal centre of mass comSum / nMolMe of the
+// step's record is one sample.  Each staggered buffer (one thread each) unwraps the sample next to
+// its previous true position by Nint of the box (ComputeDefs.h:64-65) and stores |r_true - origin|^2
+// at its count; a buffer that is full adds into the running average, in buffer order, and every
+// limitAv completed buffers the average becomes a table D_j = av_j / (6 stepDiffuse dt limitAv j)
+// in the ring `out` (tables counts them).  Counts start at -nb nVal / nBuff (InitDiffusion).
+__device__ __forceinline__ double nint64(double x) { return x < 0. ? -(double)(int)(0.5 - x) : (double)(int)(0.5 + x); }
+
+__global__ __launch_bounds__(256) void k_diffuse(DiffusionArgs a, const DevState *st, const sgk_record *records,
+                                                 int nMolMe) {
+#pragma clang fp contract(off)
+    __shared__ double com[3];
+    __shared__ int cnt[kMaxDiffuseBuffers];
+    __shared__ int cav, ntab;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        const sgk_record &r = records[st->rec - 1];
+        for (int k = 0; k < 3; ++k) com[k] = r.comSum[k] / (double)nMolMe;
+        cav = a.state[0];
+        ntab = a.state[1];
+    }
+    if (t < a.nBuff) cnt[t] = a.count[t];
+    __syncthreads();
+    if (t < a.nBuff) {
+        double *org = a.org + 3 * t, *tr = a.tru + 3 * t;
+        if (cnt[t] == 0)
+            for (int k = 0; k < 3; ++k) org[k] = tr[k] = com[k];
+        if (cnt[t] >= 0) {
+            double acc = 0.;
+            for (int k = 0; k < 3; ++k) {
+                const double L = a.L[k];
+                double dr = tr[k] - com[k];
+                dr = dr / L;
+                dr = nint64(dr);
+                dr = dr * L;
+                const double u = com[k] + dr;
+                tr[k] = u;
+                const double e = u - org[k];
+                acc += e * e;
+            }
+            a.rr[(size_t)t * a.nVal + cnt[t]] = acc;
+        }
+        ++cnt[t];
+    }
+    __syncthreads();
+    for (int nb = 0; nb < a.nBuff; ++nb) {
+        if (cnt[nb] != a.nVal) continue;  // shared value: the branch is uniform
+        for (int j = t; j < a.nVal; j += blockDim.x) a.av[j] = a.av[j] + a.rr[(size_t)nb * a.nVal + j];
+        __syncthreads();
+        if (t == 0) {
+            cnt[nb] = 0;
+            ++cav;
+        }
+        __syncthreads();
+        if (cav == a.limitAv) {
+            const double fac = 1. / ((double)(6 * a.stepDiffuse) * a.deltaT * (double)a.limitAv);
+            double *out = a.out + (size_t)(ntab % a.cap) * a.nVal;
+            for (int j = t; j < a.nVal; j += blockDim.x) {
+                out[j] = j >= 1 ? a.av[j] * (fac / (double)j) : a.av[j];
+                a.av[j] = 0.;
+            }
+            __syncthreads();
+            if (t == 0) {
+                ++ntab;
+                cav = 0;
+            }
+            __syncthreads();
+        }
+    }
+    if (t < a.nBuff) a.count[t] = cnt[t];
+    if (t == 0) {
+        a.state[0] = cav;
+        a.state[1] = ntab;
+    }
+}
+
+// Sets the buffers to their start (InitDiffusion + ZeroDiffusion, SurfaceGrowth.cu:2865-2881).
+__global__ __launch_bounds__(256) void k_diffuse_reset(DiffusionArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.nBuff) a.count[t] = -t * a.nVal / a.nBuff;
+    for (int j = t; j < a.nVal; j += gridDim.x * blockDim.x) a.av[j] = 0.;
+    if (t == 0) {
+        a.state[0] = 0;
+        a.state[1] = 0;
+    }
+}
+
 __global__ void k_reset_records(DevState *st) { st->rec = 0; }
 
 __global__ void kb_reset_records(DevState *st, int nrep) {
@@ -395,6 +483,19 @@ void launch_shear(sgk_ctx *c) {
 }
 
 void launch_reset_records(sgk_ctx *c) { hipLaunchKernelGGL(k_reset_records, dim3(1), dim3(1), 0, c->stream, c->state); }
+
+void launch_diffuse(sgk_ctx *c) {
+    if (!c->diff.count || c->dp.nMolMe == 0) return;
+    profile_begin(c);
+    hipLaunchKernelGGL(k_diffuse, dim3(1), dim3(256), 0, c->stream, c->diff, c->state, c->records, c->dp.nMolMe);
+    profile_end(c, "diffuse");
+}
+
+void launch_diffuse_reset(sgk_ctx *c) {
+    if (!c->diff.count) return;
+    hipLaunchKernelGGL(k_diffuse_reset, dim3(blocks_for(std::max(c->diff.nVal, c->diff.nBuff), 256)), dim3(256), 0,
+                       c->stream, c->diff);
+}
 
 // ---- batched launchers -----------------------------------------------------------------------
 void launch_batch_lf1(sgk_batch *b, bool kicked) {

@@ -237,11 +237,27 @@ class Engine:
         self._check(self.lib.sgk_rdf_get(self.h, _ptr(h)), "sgk_rdf_get")
         return h
 
+    def diffusion_enable(self, max_tables=64):
+        """Accumulate the centre-of-mass diffusion on the device in FP64 during run()."""
+        self._check(self.lib.sgk_diffusion_enable(self.h, max_tables), "sgk_diffusion_enable")
+
+    def diffusion_tables(self, max_tables=64):
+        """(completed tables so far, array (k, nValDiffuse) of the most recent k <= max_tables)."""
+        nv = self.params.nValDiffuse
+        out = np.zeros((max_tables, nv))
+        n = C.c_int()
+        self._check(self.lib.sgk_diffusion_get(self.h, _ptr(out), max_tables, C.byref(n)), "sgk_diffusion_get")
+        return n.value, out[: min(n.value, max_tables)]
+
     # -- device-resident loop ------------------------------------------------------------------
     def run(self, nsteps, rdf_every=0, records=True):
         recs = (N.SgkRecord * nsteps)() if records else None
         self._check(self.lib.sgk_run(self.h, nsteps, rdf_every, recs), "sgk_run")
         return [record_dict(r) for r in recs] if records else None
+
+    def prepare_run(self, nsteps, rdf_every=0):
+        """Capture the graph of the next run(nsteps, rdf_every) now, so that run does not pay for it."""
+        self._check(self.lib.sgk_prepare_run(self.h, nsteps, rdf_every), "sgk_prepare_run")
 
     def last_run_ms(self):
         return self.lib.sgk_last_run_ms(self.h)
