@@ -285,6 +285,32 @@ long long sgk_batch_device_bytes(const sgk_batch *b);
 int sgk_batch_metal_lanes(const sgk_batch *b);
 int sgk_batch_atoms(const sgk_batch *b);
 
+/* ---- force evaluation without allocation ---------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  It enqueues, on `stream`
+ * (a hipStream_t; NULL = the null stream), what sgk_create + sgk_upload + sgk_bin + sgk_build_nebr +
+ * sgk_compute_forces do on a fresh context, with the same kernels in the same order, so its results
+ * equal theirs bit for bit.  It calls no hipMalloc, hipFree, memset, copy or synchronisation.
+ *   pos   device, nMol x {x,y,z,w}: double4 (precision SG_PREC_MIXED) or float4 (SG_PREC_FP32); w
+ *         receives the per-atom energy, as the reference's ComputeForcesK writes dr[i].w
+ *   vel   device, 3*nMol in the storage precision, SoA (x block, y block, z block), or NULL (zero);
+ *         read for the block reductions only
+ *   acc   device, 3*nMol in the storage precision, SoA: the accelerations
+ *   fric  device, nMolMe floats: metal-carbon friction x, or NULL
+ *   flags device, one int: the SGK_FLAG_* bits of the evaluation (capacity overflows), or NULL
+ * Intermediate buffers (cells, lists, densities, partials, run state) are carved from one
+ * static device array of SGK_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; larger
+ * systems are rejected with SGK_E_UNSUPPORTED (sgk_noalloc_scratch_bytes gives a system's need).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU race.  keep_full_nl is
+ * not supported.  Errors: the return code, with the reason from sgk_noalloc_last_error(). */
+#define SGK_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int sgk_forces_noalloc(const SimParams *sp, const sgk_config *cfg, void *pos, const void *vel, void *acc,
+                       float *fric, int *flags, void *stream);
+/* Scratch bytes sgk_forces_noalloc needs for this system (with fric and vel NULL); -1 if invalid. */
+long long sgk_noalloc_scratch_bytes(const SimParams *sp, const sgk_config *cfg);
+/* Reason for the last failed sgk_forces_noalloc on this thread. */
+const char *sgk_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,7 @@ SGK_OBSERVABLES = ("adhesion", "height")                    # SGK_OBS_*
 
 SG_PREC_FP32, SG_PREC_MIXED = 0, 1
 SGK_E_ARG, SGK_E_HIP, SGK_E_NOMEM, SGK_E_OVERFLOW, SGK_E_UNSUPPORTED, SGK_E_STATE, SGK_E_DISPLACEMENT = range(-1, -8, -1)
+SGK_NOALLOC_SCRATCH_BYTES = 64 << 20  # sgk.h
 SG_GEOM_REFERENCE, SG_GEOM_SPHERE, SG_GEOM_ICOSAHEDRON = 0, 1, 2
 SG_VEL_REFERENCE, SG_VEL_MAXWELL = 0, 1
 SGK_FLAGS = {1: "cell overflow", 2: "nl overflow", 4: "lj overflow", 8: "close overflow",
@@ -166,7 +167,7 @@ EXPORTED_SYMBOLS = [
     "sgk_batch_rdf_get", "sgk_batch_rdf_clear", "sgk_batch_device_bytes", "sgk_batch_metal_lanes",
     "sgk_batch_atoms",
     "sgk_adjoint_forward", "sgk_adjoint_backward", "sgk_adjoint_get_info", "sgk_force_vjp",
-    "sgk_adhesion_energy", "sgk_prepare_run", "sgk_diffusion_enable", "sgk_diffusion_get",
+    "sgk_adhesion_energy", "sgk_prepare_run", "sgk_diffusion_enable", "sgk_diffusion_get", "sgk_forces_noalloc", "sgk_noalloc_scratch_bytes", "sgk_noalloc_last_error",
 ]
 
 _lib = None
@@ -201,6 +202,9 @@ def _proto(lib):
         "sgk_diffusion_enable": (_i, [vp, _i]),
         "sgk_prepare_run": (_i, [vp, _i, _i]),
         "sgk_diffusion_get": (_i, [vp, vp, _i, P(_i)]),
+        "sgk_forces_noalloc": (_i, [P(SimParams), P(SgkConfig), vp, vp, vp, vp, vp, vp]),
+        "sgk_noalloc_scratch_bytes": (C.c_longlong, [P(SimParams), P(SgkConfig)]),
+        "sgk_noalloc_last_error": (C.c_char_p, []),
         "sgk_destroy": (_i, [vp]),
         "sgk_last_error": (C.c_char_p, [vp]),
         "sgk_device_count": (_i, []),
