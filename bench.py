@@ -2,7 +2,10 @@
 
 A step is one full MD step of the hot path (leapfrog + boundary + binning when due, neighbour list
 when due, EAM density, fused EAM/LJ/graphene force pass with leapfrog part 2 and the step
-reductions, thermostat on its steps) over the whole system, with all state resident in HBM.
+reductions, thermostat on its steps) over the whole system, with all state resident in HBM, plus
+the samples the large workload takes on the device: the radial distribution function (400 bins to
+12 A, every metal pair within range) and the centre-of-mass MSD buffers, each every --sample-every
+steps (default 10).  The step graph of the timed pattern is captured before the timed region.
 
 Default workload: the large system - 1,003,520-atom periodic graphene sheet + 100,000-atom Ni fcc
 sphere 2.5 A above it, 1,000 K Maxwell-Boltzmann velocities, dt = 1 fs, Berendsen tau = 0.1 ps,
@@ -11,11 +14,14 @@ form's atom-steps/s; the plain FP32 form is timed on the same workload and repor
 
 Roofline: the dominant kernel is the force pass; its algorithmic bytes per launch are the
 per-atom-step figure of the state traffic (plain 48 + 20 f_Me B, mixed 96 + 20 f_Me B) times the
-atoms, divided by its mean launch duration from HIP events on the engine stream.
+atoms, divided by its mean launch duration from HIP events on the engine stream.  `bound` is the
+larger of the pass's measured HBM-traffic fraction and VALU-issue fraction (committed rocprofv3
+counter summaries); `step_frac` is the same algorithmic bytes over the whole step time.
 
-cpu_baseline: the FP32 CPU restatement of the same step (oracle/, single thread) on a bounded
-number of steps of the same workload.  Its FP64 twin is the checker for the force error of both
-GPU forms, on the initial state and on the state the timed mixed run ends in.
+cpu_baseline: the FP32 CPU restatement of the same step (oracle/), built for speed (-O3
+-march=x86-64-v3, OpenMP over the per-atom loops), on one core and on the host's cores, each on a
+bounded number of steps of the same workload.  The FP64 oracle is the checker for the force and
+energy error of both GPU forms, on the initial state and on the state the timed mixed run ends in.
 """
 import argparse
 import json
@@ -195,25 +201,32 @@ def b_alg(precision, f_me):
     return (96.0 if precision == "mixed" else 48.0) + 20.0 * f_me
 
 
-def time_form(sys_, precision, steps, warmup, rebuild, chunk):
+def time_form(sys_, precision, steps, warmup, rebuild, chunk, sample_every):
+    """Warm-up, then `steps` timed device-resident steps (RDF and MSD samples every `sample_every`
+    steps, complete-range RDF), then the same number again with HIP events around every launch for
+    the per-kernel durations.  The graph of the timed chunk is captured after the warm-up."""
     import surfacegrowthconsole_amd as sg
 
     p = sys_.params
-    eng = sg.Engine(p, precision=precision, rebuild_every=rebuild, records_capacity=chunk)
+    eng = sg.Engine(p, precision=precision, rebuild_every=rebuild, records_capacity=chunk, rdf_complete=1)
     eng.upload(sys_.r, sys_.v, sys_.a)
     eng.set_run_state(0, 1, 0.0)
+    if sample_every:
+        eng.diffusion_enable(4)
+    eng.rdf_clear()
     done = 0
     while done < warmup:
         n = min(chunk, warmup - done)
-        eng.run(n, records=False)
+        eng.run(n, sample_every, records=False)
         done += n
+    eng.prepare_run(min(chunk, steps), sample_every)
     # timed region: `steps` device-resident steps; sgk_run synchronises the stream on return
     t0 = time.perf_counter()
     dev_ms = 0.0
     done = 0
     while done < steps:
         n = min(chunk, steps - done)
-        eng.run(n, records=False)
+        eng.run(n, sample_every, records=False)
         dev_ms += eng.last_run_ms()
         done += n
     wall = time.perf_counter() - t0
@@ -222,7 +235,7 @@ def time_form(sys_, precision, steps, warmup, rebuild, chunk):
     done = 0
     while done < steps:
         n = min(chunk, steps - done)
-        eng.run(n, records=False)
+        eng.run(n, sample_every, records=False)
         done += n
     stats = eng.kernel_stats()
     eng.set_profiling(False)
@@ -244,13 +257,15 @@ def gpu_forces(params, r, form):
         eng.bin()
         eng.build_nebr()
         eng.compute_forces()
-        acc, _, _ = eng.forces()
-    return acc
+        acc, e, _ = eng.forces()
+    return acc, e
 
 
 def force_errors(params, states):
-    """e_F = max_i |F_i - F_i^64| / rms_j |F_j^64| of both forms against the FP64 oracle, per state.
-    The plain form is evaluated at the float-rounded positions it stores, its reference too."""
+    """Per state and form, against the FP64 oracle on the same positions: e_F = max_i |F_i - F_i^64| /
+    rms_j |F_j^64|, the global ||dF||_2 / ||F^64||_2 and the potential-energy error |E - E^64| / |E^64|
+    (E = sum of the per-atom energies).  The plain form is evaluated at the float-rounded positions it
+    stores, its reference too."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -262,32 +277,65 @@ def force_errors(params, states):
         for prec in ("mixed", "mixed_fp32pairs", "fp32"):
             rin = np.asarray(r, np.float64) if prec != "fp32" else np.asarray(r, np.float32).astype(np.float64)
             _, nn, nbl = o64.neighbour_list(rin)
-            a64, _, _ = o64.forces(rin, nn, nbl)
-            acc = gpu_forces(params, rin, prec)
+            a64, e64, _ = o64.forces(rin, nn, nbl)
+            acc, e = gpu_forces(params, rin, prec)
             rms = float(np.sqrt((a64 ** 2).sum(1).mean()))
-            out[f"{prec}_{label}"] = float(np.abs(acc - a64).max() / rms)
-            log(f"[bench] e_F {prec} {label}: {out[f'{prec}_{label}']:.3e}")
+            key = f"{prec}_{label}"
+            out[key] = float(np.abs(acc - a64).max() / rms)
+            out[key + "_l2"] = float(np.sqrt(((acc - a64) ** 2).sum()) / np.sqrt((a64 ** 2).sum()))
+            out[key + "_energy"] = float(abs(e.sum() - e64.sum()) / abs(e64.sum()))
+            log(f"[bench] {prec} {label}: e_F {out[key]:.3e}, l2 {out[key + '_l2']:.3e}, "
+                f"energy {out[key + '_energy']:.3e}")
     return out
 
 
+def host_cpu():
+    """Model name and core counts of the host this runs on."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(sys_, rebuild, seconds):
-    """FP32 CPU restatement (oracle) timed on the host, single thread, on a bounded sample of steps."""
+    """The FP32 CPU restatement of the step built for speed (oracle/_build/liboracle_omp.so: -O3
+    -march=x86-64-v3, OpenMP per-atom loops), timed on the host: one thread, then all the threads
+    OpenMP is given (OMP_NUM_THREADS, else the cores this process may use), each on about `seconds`
+    of steps of the same workload after one untimed step.  Rebuild cadence as the GPU run."""
+    import ctypes
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     p = sys_.params
-    o32 = oracle.Oracle(p, "f32")
-    # one step to size the sample, then ~`seconds` of steps
-    t0 = time.perf_counter()
-    err, r, v, a, _ = o32.run(sys_.r, sys_.v, sys_.a, 1, 0, 1, 0.0, rebuild)
-    t1 = time.perf_counter() - t0
-    n = max(2, min(200, int(seconds / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    err, r, v, a, _ = o32.run(r, v, a, n, 1, 0, 0.0, rebuild)
-    dt = time.perf_counter() - t0
-    rate = p.nMol * n / dt
-    return {"value": rate, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps (after 1 untimed) of the same workload, FP32 C restatement, 1 thread"}
+    lib = oracle.timing_lib()
+    gomp = ctypes.CDLL("libgomp.so.1")
+    model, ncpu, nallowed = host_cpu()
+    threads_all = int(os.environ.get("OMP_NUM_THREADS") or nallowed)
+    res = {}
+    for threads in (1, threads_all):
+        gomp.omp_set_num_threads(threads)
+        o = oracle.Oracle(p, "f32", library=lib)
+        t0 = time.perf_counter()
+        err, r, v, a, _ = o.run(sys_.r, sys_.v, sys_.a, 1, 0, 1, 0.0, rebuild)
+        t1 = time.perf_counter() - t0
+        n = max(2, min(200, int(seconds / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        err, r, v, a, _ = o.run(r, v, a, n, 1, 0, 0.0, rebuild)
+        dt = time.perf_counter() - t0
+        res[threads] = (p.nMol * n / dt, n)
+        log(f"[bench] cpu {threads} thread(s): {res[threads][0]:.4e} atom-steps/s ({n} steps)")
+    one, allc = res[1], res[threads_all]
+    return {"value": one[0], "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{one[1]} steps (after 1 untimed) of the same workload, FP32 restatement built -O3 "
+                      f"-march=x86-64-v3, 1 thread",
+            "all_cores": {"value": allc[0], "cores": threads_all, "sample": f"{allc[1]} steps, OpenMP"},
+            "host": {"cpu": model, "nproc": ncpu, "cores_allowed": nallowed}}
 
 
 def bench_adjoint(equil, window, every, fd_steps):
@@ -382,6 +430,8 @@ def main():
     ap.add_argument("--config", default="large", choices=["small", "medium", "large", "icosa"])
     ap.add_argument("--variant", default="config", choices=["config", "reference"])
     ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--sample-every", type=int, default=10,
+                    help="RDF and centre-of-mass MSD sample interval in the timed steps (0: none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -424,23 +474,40 @@ def main():
     from surfacegrowthconsole_amd.configs import rebuild_every
 
     log(f"[bench] building {args.config}/{args.variant}")
-    s = sg.config_system(args.config, args.variant)
+    extra = {"stepDiffuse": args.sample_every} if args.sample_every else {}
+    s = sg.config_system(args.config, args.variant, **extra)
     p = s.params
     rebuild = rebuild_every(args.variant)
     f_me = p.nMolMe / p.nMol
     res = {}
     for prec in ("mixed", "fp32"):
         log(f"[bench] {prec}: {args.warmup} warm-up + {args.steps} timed steps")
-        wall, dev_ms, stats, r_end = time_form(s, prec, args.steps, args.warmup, rebuild, args.chunk)
+        wall, dev_ms, stats, r_end = time_form(s, prec, args.steps, args.warmup, rebuild, args.chunk,
+                                               args.sample_every)
         res[prec] = dict(wall=wall, dev_ms=dev_ms, stats=stats, r_end=r_end)
         log(f"[bench] {prec}: {wall / args.steps * 1e3:.3f} ms/step, {p.nMol * args.steps / wall:.4e} atom-steps/s")
     m = res["mixed"]
     value = p.nMol * args.steps / m["wall"]
+    ms_per_step = m["wall"] / args.steps * 1e3
     fl, fms = m["stats"]["force"]
     force_us = fms / fl * 1e3
-    alg_bytes = b_alg("mixed", f_me) * p.nMol
+    alg_per_atom = b_alg("mixed", f_me)
+    alg_bytes = alg_per_atom * p.nMol
     achieved = alg_bytes / (force_us * 1e-6) / 1e9
+    traffic = measured_traffic(args.config)
+    hbm_frac = traffic / (force_us * 1e-6) / 1e9 / HBM_PEAK_GBS if traffic else None
+    valu_us = valu_issue_us(args.config)
+    valu_frac = valu_us / force_us if valu_us is not None else None
+    bound = "hbm"
+    if valu_frac is not None and (hbm_frac is None or valu_frac > hbm_frac):
+        bound = "valu"
     kernels = {k: {"launches": v[0], "us_per_launch": round(v[1] / v[0] * 1e3, 3)} for k, v in m["stats"].items()}
+    samples = {}
+    for k in ("rdf", "diffuse"):
+        if k in m["stats"]:
+            samples[k] = {"us_per_sample": round(m["stats"][k][1] / m["stats"][k][0] * 1e3, 3),
+                          "every_steps": args.sample_every}
+    lu = p.lengthU  # angstrom per reduced length unit
     out = {
         "metric": "atom-steps/s on 1 MI355X, mixed vs FP32 at <=1e-5 rel force err vs FP64",
         "value": value,
@@ -448,7 +515,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": m["wall"] / args.steps * 1e3,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -457,20 +524,29 @@ def main():
         "config": {"workload": f"{args.config}: {p.nMol - p.nMolMe} C graphene + {p.nMolMe} Ni fcc sphere, "
                                f"{p.temperature * p.temperatureU:.0f} K, dt 1 fs, variant {args.variant}",
                    "atoms": p.nMol, "metal_fraction": round(f_me, 4), "precision": "mixed",
-                   "rebuild_every": rebuild, "parallelism": "single GPU"},
+                   "rebuild_every": rebuild,
+                   "eam_cutoff_A": round(p.rCutEam * lu, 3),
+                   "lj_cutoff_A": round(min(p.rCutLJ, math.sqrt(p.rrNebr)) * lu, 3),
+                   "cutoffs_note": "reference rules: EAM cut at 1.45 a; LJ limited by the neighbour-list radius "
+                                   "(rCutEam + skin), not 2.5 sigma",
+                   "rdf": f"{p.sizeHistRdf} bins to {p.rangeRdf * lu:.1f} A, every metal pair in range, "
+                          f"every {args.sample_every} steps" if args.sample_every else "none",
+                   "msd": f"centre-of-mass buffers on the device (FP64), every {args.sample_every} steps"
+                          if args.sample_every else "none",
+                   "parallelism": "single GPU"},
         "fp32": {"value": p.nMol * args.steps / res["fp32"]["wall"],
                  "ms_per_step": res["fp32"]["wall"] / args.steps * 1e3},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config),
+        "roofline": {"bound": bound, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_force (fused metal+carbon force pass)", "us_per_launch": force_us,
-                     "alg_bytes_per_atom_step": b_alg("mixed", f_me),
-                     # the pass is instruction-bound: share of its launch time the SIMDs spend issuing VALU
-                     "valu_issue_frac": (valu_issue_us(args.config) / force_us
-                                         if valu_issue_us(args.config) is not None else None)},
+                     "alg_bytes_per_atom_step": alg_per_atom,
+                     "hbm_traffic_frac": hbm_frac, "valu_issue_frac": valu_frac,
+                     "step_frac": alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "samples": samples,
         "kernels": kernels,
     }
     if not args.no_check:
-        log("[bench] force error against the FP64 oracle (initial state, state after the mixed run)")
+        log("[bench] force and energy error against the FP64 oracle (initial state, state after the mixed run)")
         out["force_error_vs_fp64"] = force_errors(p, {"initial": s.r, "final": res["mixed"]["r_end"]})
     if not args.no_cpu:
         log("[bench] cpu baseline")

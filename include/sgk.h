@@ -68,6 +68,10 @@ typedef struct sgk_config {
                              2 LDS-staged per workgroup, 3 (default) lanes per metal atom, atoms
                              visited in cell order (sgk_build_nl_order); every scheme gives the same
                              lists */
+    int rdf_complete;     /* RDF: 0 the reference's 27-cell stencil of the last rebuilt cells (complete
+                             only to one cell width); 1 every metal-metal pair within rangeRdf at the
+                             current positions (minimum image; rangeRdf must stay below half the box).
+                             No counterpart in the reference. */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with

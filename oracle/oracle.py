@@ -49,11 +49,26 @@ class OrcRecord(C.Structure):
                 ("bboxMin", _d * 3), ("bboxMax", _d * 3), ("dispHi", _d)]
 
 
+OMP_LIB = os.path.join(HERE, "_build", "liboracle_omp.so")  # CPU-baseline timing build (FP32, OpenMP)
+
 _lib = None
+_omp = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def timing_lib():
+    """The FP32 restatement built for speed (-O3 -march=x86-64-v3, OpenMP per-atom loops): the CPU
+    baseline's timing library.  Its *_f32 functions have the signatures of lib()'s."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(OMP_LIB):
+            build()
+        _omp = C.CDLL(OMP_LIB)
+        _declare(_omp, ("f32",))
+    return _omp
 
 
 def lib():
@@ -62,29 +77,8 @@ def lib():
         if not os.path.exists(LIB):
             build()
         _lib = C.CDLL(LIB)
+        _declare(_lib, ("f32", "f64"))
         vp = C.c_void_p
-        for sfx in ("f32", "f64"):
-            R = C.c_float if sfx == "f32" else C.c_double
-            spec = {
-                "orc_bin": (C.c_int, [vp, vp, vp, vp, vp]),
-                "orc_nebr": (C.c_int, [vp, vp, vp, vp, vp, vp]),
-                "orc_rho": (None, [vp, vp, vp, vp, vp]),
-                "orc_forces": (None, [vp, vp, vp, vp, vp, vp, vp]),
-                "orc_forces_virial": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
-                "orc_embed": (R, [vp, R, vp]),
-                "orc_pair": (None, [vp, R, vp]),
-                "orc_leapfrog": (None, [vp, C.c_int, vp, vp, vp]),
-                "orc_thermostat": (None, [vp, vp, C.c_double, C.c_int]),
-                "orc_props": (None, [vp, vp, vp, vp, vp]),
-                "orc_rdf": (C.c_int, [vp, vp, vp, vp, vp, vp]),
-                "orc_diffusion_sample": (C.c_int, [vp, vp, vp]),
-                "orc_run": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
-                "orc_adhesion": (R, [vp, vp, vp, vp]),
-                "orc_window": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
-            }
-            for name, (res, args) in spec.items():
-                fn = getattr(_lib, f"{name}_{sfx}")
-                fn.restype, fn.argtypes = res, args
         _lib.orc_diffusion_create.restype = vp
         _lib.orc_diffusion_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp]
         _lib.orc_diffusion_destroy.argtypes = [vp]
@@ -93,6 +87,34 @@ def lib():
         _lib.orc_tangent_forces.restype = None
         _lib.orc_tangent_forces.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     return _lib
+
+
+def _declare(L, suffixes):
+    """ctypes signatures of the precision-suffixed functions present in library L."""
+    vp = C.c_void_p
+    for sfx in suffixes:
+        R = C.c_float if sfx == "f32" else C.c_double
+        spec = {
+            "orc_bin": (C.c_int, [vp, vp, vp, vp, vp]),
+            "orc_nebr": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+            "orc_rho": (None, [vp, vp, vp, vp, vp]),
+            "orc_forces": (None, [vp, vp, vp, vp, vp, vp, vp]),
+            "orc_forces_virial": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
+            "orc_embed": (R, [vp, R, vp]),
+            "orc_pair": (None, [vp, R, vp]),
+            "orc_leapfrog": (None, [vp, C.c_int, vp, vp, vp]),
+            "orc_thermostat": (None, [vp, vp, C.c_double, C.c_int]),
+            "orc_props": (None, [vp, vp, vp, vp, vp]),
+            "orc_rdf": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+            "orc_diffusion_sample": (C.c_int, [vp, vp, vp]),
+            "orc_run": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
+            "orc_adhesion": (R, [vp, vp, vp, vp]),
+            "orc_window": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
+        }
+        for name, (res, args) in spec.items():
+            fn = getattr(L, f"{name}_{sfx}", None)
+            if fn is not None:
+                fn.restype, fn.argtypes = res, args
 
 
 def _p(a):
@@ -123,8 +145,8 @@ def params_from(sp, cell_cap=64):
 class Oracle:
     """One precision of the oracle ("f32" restatement or "f64" reference-of-record)."""
 
-    def __init__(self, sp, prec="f64", cell_cap=64):
-        self.l = lib()
+    def __init__(self, sp, prec="f64", cell_cap=64, library=None):
+        self.l = library if library is not None else lib()
         self.sfx = prec
         self.dt = np.float32 if prec == "f32" else np.float64
         self.p = params_from(sp, cell_cap)

@@ -11,6 +11,16 @@
  */
 #include "oracle.h"
 
+/* OMP_FOR: the per-atom (or per-cell) loops whose iterations write only their own atom's outputs run
+ * in parallel in the CPU-baseline timing build (-fopenmp); every other build ignores it. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(dynamic, 256)")
+#define OMP_FOR_ERR _Pragma("omp parallel for schedule(dynamic, 256) reduction(| : err)")
+#else
+#define OMP_FOR
+#define OMP_FOR_ERR
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -126,6 +136,7 @@ int FN(orc_nebr)(const orc_params *p, const REAL *pos4, const int *count, const 
     int err = 0;
     for (size_t q = 0; q < (size_t)nMol * p->iNebrMax; ++q) NBL[q] = -1;
     for (int i = 0; i < nMol; ++i) NN[i] = 0;
+    OMP_FOR_ERR
     for (int c = 0; c < cells[0] * cells[1] * cells[2]; ++c) {
         const int ccz = c / (cells[0] * cells[1]);
         const int ccy = (c - ccz * cells[0] * cells[1]) / cells[0];
@@ -239,6 +250,7 @@ void FN(orc_pair)(const orc_params *p, REAL r, REAL *out4) {
 /* EamComputeRhoK (.cu:716-773): rho_i = sum over metal neighbours with r^2 < rrCutEam. */
 void FN(orc_rho)(const orc_params *p, const REAL *pos4, const int *NN, const int *NBL, REAL *rho) {
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
+    OMP_FOR
     for (int i = 0; i < p->nMolMe; ++i) {
         REAL sum = 0;
         for (int j = 0; j < NN[i]; ++j) {
@@ -272,6 +284,7 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
     const REAL z0 = p->z_0, rrC = p->rrCutC, mu_r = p->mu_r, mu_p = p->mu_p, mu_t = p->mu_theta,
                th0 = p->theta_0;
+    OMP_FOR
     for (int i = 0; i < nMol; ++i) {
         const int metal = i < nMe;
         REAL fx = 0, fy = 0, fz = 0, frx = 0, uSum = 0, wSum = 0;
@@ -428,6 +441,7 @@ void FN(orc_leapfrog)(const orc_params *p, int part, REAL *pos4, REAL *vel, cons
     const float xLeft = 0.5f * (cos30 - p->region[0]), xRight = 0.5f * (p->region[0] - cos30);
     const float yBottom = 0.5f * (1.f - p->region[1]), yTop = 0.5f * (p->region[1] - 1.f);
     const REAL hdt = K(0.5) * (REAL)p->deltaT, dt = p->deltaT;
+    OMP_FOR
     for (int n = 0; n < p->nMol; ++n) {
         REAL *r = pos4 + 4 * n, *v = vel + 3 * n;
         const REAL *a = acc + 3 * n;

@@ -369,6 +369,20 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     }
     const size_t nParts = (size_t)c->part2Off;
     SGK_TRY(dalloc(c, (char **)&c->partials, (nParts + nParts / 64 + 2) * sgk::partial_bytes(c), true));
+    if (cfg.rdf_complete && p.nMolMe > 0) {
+        c->rdfGrid = sgk::rdf_grid(p);
+        if (c->rdfGrid.ncell == 0) {
+            g_create_error = "rdf_complete needs rangeRdf below half of every box edge";
+            sgk_destroy(c);
+            return SGK_E_UNSUPPORTED;
+        }
+        const size_t nb = (size_t)c->rdfGrid.ncell + 1;
+        SGK_TRY(dalloc(c, &c->rdfCnt, nb, false));
+        SGK_TRY(dalloc(c, &c->rdfBlock, (nb + 4095) / 4096, false));
+        SGK_TRY(dalloc(c, &c->rdfKey, nMe, false));
+        SGK_TRY(dalloc(c, &c->rdfSlot, nMe, false));
+        SGK_TRY(dalloc(c, &c->rdfPos, nMe, false));
+    }
     SGK_TRY(dalloc(c, &c->state, 1, true));
     SGK_TRY(dalloc(c, &c->records, (size_t)cfg.records_capacity, true));
     SGK_TRY(dalloc(c, &c->hist, (size_t)p.sizeHistRdf, true));
@@ -407,7 +421,8 @@ int sgk_destroy(sgk_ctx *c) {
     void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
                     c->cnt[0], c->cnt[1], c->slots[0], c->slots[1], c->slotOf, c->cellPos, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
                     c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist,
-                    c->nlOrder, c->ordCnt, c->ordKey, c->ordSlot, c->ordBlock};
+                    c->nlOrder, c->ordCnt, c->ordKey, c->ordSlot, c->ordBlock, c->rdfCnt, c->rdfBlock, c->rdfKey,
+                    c->rdfSlot, c->rdfPos};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (c->diffMem) (void)hipFree(c->diffMem);

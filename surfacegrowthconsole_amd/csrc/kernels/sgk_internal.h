@@ -79,6 +79,10 @@ struct sgk_ctx {
     sgk::Adjoint *adj = nullptr;                        // adjoint storage (sgk_adjoint_*), on demand
     sgk::DiffusionArgs diff = {};                       // device diffusion (sgk_diffusion_enable), on demand
     void *diffMem = nullptr;
+    // complete-range RDF (cfg.rdf_complete): metal atoms sorted by the RDF grid at each sample
+    sgk::RdfGrid rdfGrid = {};
+    int *rdfCnt = nullptr, *rdfKey = nullptr, *rdfSlot = nullptr, *rdfBlock = nullptr;
+    float4 *rdfPos = nullptr;
 };
 
 // A batch of replicas advanced together (sgk_batch_*): shared arrays, one range per replica.
@@ -173,6 +177,7 @@ void build_nl_order(sgk_ctx *c);
 constexpr int kNlOrderSteps = 100;  // age (steps) at which sgk_run rebuilds the nl_scheme 3 list-build order
 void launch_commit_rebuild(sgk_ctx *c);
 void launch_rdf(sgk_ctx *c);
+RdfGrid rdf_grid(const DevParams &p);  // grid of the complete-range RDF (ncell 0: range too large)
 // sgk_force.hip
 void launch_rho(sgk_ctx *c);
 void launch_batch_rho(sgk_batch *b);

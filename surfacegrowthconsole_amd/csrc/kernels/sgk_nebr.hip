@@ -1035,8 +1035,130 @@ __global__ __launch_bounds__(256) void kb_rdf(BatchArgs a) {
                 a.hist + (size_t)e.x * B.p.sizeHistRdf, sh, &a.state[e.x].errors);
 }
 
+// ---- complete-range RDF (no counterpart in the reference) ------------------------------------
+// The reference stencil above finds every pair only up to one cell width.  This variant counts every
+// ordered metal-metal pair within rangeRdf at the current positions: metal atoms are sorted into a
+// grid of cells at least rangeRdf/2 wide (count, scan, scatter of float positions with the atom index
+// in w), then each atom's lanes walk the 5x5x5 cells around it.  A pair is counted once, from the atom
+// in the lower cell (or, in one cell, the lower index), and adds 2 to its bin - the reference's
+// histogram counts both orders.  Displacements use the single-image wrap, so rangeRdf must stay below
+// half the box; bins use the reference's formula.
+__device__ __forceinline__ int rdf_cell_axis(float x, float L, float iw, int n) {
+    const int c = (int)((x + 0.5f * L) * iw);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_rdfc_count(DevParams p, RdfGrid g, const typename Prec<M>::P4 *__restrict__ pos,
+                                                    int *__restrict__ cnt, int *__restrict__ key, int *__restrict__ slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nMolMe) return;
+    const float3 q = f3(pos[i]);
+    const int c = (rdf_cell_axis(q.z, p.Lz, g.iwz, g.nz) * g.ny + rdf_cell_axis(q.y, p.Ly, g.iwy, g.ny)) * g.nx +
+                  rdf_cell_axis(q.x, p.Lx, g.iwx, g.nx);
+    key[i] = c;
+    slot[i] = atomicAdd(&cnt[c], 1);
+}
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_rdfc_scatter(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
+                                                      const int *__restrict__ start, const int *__restrict__ key,
+                                                      const int *__restrict__ slot, float4 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nMolMe) return;
+    const float3 q = f3(pos[i]);
+    out[start[key[i]] + slot[i]] = make_float4(q.x, q.y, q.z, __int_as_float(i));
+}
+
+__device__ __forceinline__ float wrap_once(float d, float L) {
+    return d >= 0.5f * L ? d - L : (d < -0.5f * L ? d + L : d);
+}
+
+constexpr int kRdfLanes = 8;
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_rdfc_pairs(DevParams p, RdfGrid g, const typename Prec<M>::P4 *__restrict__ pos,
+                                                    const int *__restrict__ start, const int *__restrict__ key,
+                                                    const float4 *__restrict__ sorted, unsigned *__restrict__ hist,
+                                                    int *errors) {
+#pragma clang fp contract(off)
+    extern __shared__ unsigned sh[];
+    for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
+    __syncthreads();
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / kRdfLanes, l = gt % kRdfLanes;
+    int edge = 0;
+    if (i < p.nMolMe) {
+        const float3 A = f3(pos[i]);
+        const int ci = key[i];
+        const int cz = ci / (g.nx * g.ny), cy = (ci / g.nx) % g.ny, cx = ci % g.nx;
+        // stencil extents per axis: 2s+1 cells around the atom's, or the whole axis once
+        const int tx = 2 * g.sx + 1 < g.nx ? 2 * g.sx + 1 : g.nx;
+        const int ty = 2 * g.sy + 1 < g.ny ? 2 * g.sy + 1 : g.ny;
+        const int tz = 2 * g.sz + 1 < g.nz ? 2 * g.sz + 1 : g.nz;
+        const int x0 = tx < g.nx ? cx - g.sx : 0, y0 = ty < g.ny ? cy - g.sy : 0, z0 = tz < g.nz ? cz - g.sz : 0;
+        const int ncells = tx * ty * tz;
+        const float range2 = p.rangeRdf2;
+        for (int t = l; t < ncells; t += kRdfLanes) {
+            const int ox = t % tx, oy = (t / tx) % ty, oz = t / (tx * ty);
+            int x = x0 + ox, y = y0 + oy, z = z0 + oz;
+            x += x < 0 ? g.nx : (x >= g.nx ? -g.nx : 0);
+            y += y < 0 ? g.ny : (y >= g.ny ? -g.ny : 0);
+            z += z < 0 ? g.nz : (z >= g.nz ? -g.nz : 0);
+            const int c = (z * g.ny + y) * g.nx + x;
+            if (c < ci) continue;  // this cell's pairs with i are counted from the other side
+            const int b0 = start[c], b1 = start[c + 1];
+            for (int j = b0; j < b1; ++j) {
+                const float4 e = sorted[j];
+                if (c == ci && __float_as_int(e.w) <= i) continue;
+                const float dx = wrap_once(A.x - e.x, p.Lx), dy = wrap_once(A.y - e.y, p.Ly),
+                            dz = wrap_once(A.z - e.z, p.Lz);
+                const float rr = dx * dx + dy * dy + dz * dz;
+                if (rr < range2) {
+                    const int bin = (int)(sqrtf(rr) * p.intervalRdf);
+                    if (bin >= p.sizeHistRdf)
+                        edge = 1;
+                    else
+                        atomicAdd(&sh[bin], 2u);
+                }
+            }
+        }
+    }
+    if (edge) atomicOr(errors, SGK_FLAG_RDF_EDGE);
+    __syncthreads();
+    for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x)
+        if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ a, long long n, int v) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        a[t] = v;
+}
+
 // ---- launchers ---------------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// The RDF grid of a system: cells at least rangeRdf/2 wide per axis (a pair within range then lies
+// within two cells); 0 cells when rangeRdf reaches half the box (the single-image wrap would miss
+// pairs).
+RdfGrid rdf_grid(const DevParams &p) {
+    RdfGrid g{};
+    const float range = sqrtf(p.rangeRdf2);
+    if (!(range > 0.f) || range >= 0.5f * p.Lx || range >= 0.5f * p.Ly || range >= 0.5f * p.Lz) return g;
+    // 1e-4 wider than half the range: the float rounding of a cell index (~1e-7 of the box) can then
+    // never put a pair within range three cells apart
+    const float w = 0.5f * range * 1.0001f;
+    g.nx = std::max(1, (int)(p.Lx / w));
+    g.ny = std::max(1, (int)(p.Ly / w));
+    g.nz = std::max(1, (int)(p.Lz / w));
+    // the cell index uses (x + L/2) * n / L, so each cell is at least L/n >= w wide
+    g.iwx = (float)g.nx / p.Lx;
+    g.iwy = (float)g.ny / p.Ly;
+    g.iwz = (float)g.nz / p.Lz;
+    g.sx = g.sy = g.sz = 2;
+    g.ncell = g.nx * g.ny * g.nz;
+    return g;
+}
 
 void launch_bin(sgk_ctx *c, bool force) {
     if (c->dp.nMol == 0) return;
@@ -1162,6 +1284,36 @@ void launch_rdf(sgk_ctx *c) {
     profile_begin(c);
     const unsigned g = blocks_for(c->dp.nMolMe, 256);
     const size_t shm = sizeof(unsigned) * (size_t)c->dp.sizeHistRdf;
+    if (c->rdfPos) {  // complete-range variant
+        const DevParams &p = c->dp;
+        const RdfGrid &G = c->rdfGrid;
+        const int nb = G.ncell + 1;  // the last bucket stays empty: its start is the total
+        hipLaunchKernelGGL(k_fill_i32, dim3(std::min(blocks_for(nb, 256), 4096u)), dim3(256), 0, c->stream, c->rdfCnt,
+                           (long long)nb, 0);
+        if (c->mixed)
+            hipLaunchKernelGGL(k_rdfc_count<true>, dim3(g), dim3(256), 0, c->stream, p, G, (const double4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfSlot);
+        else
+            hipLaunchKernelGGL(k_rdfc_count<false>, dim3(g), dim3(256), 0, c->stream, p, G, (const float4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfSlot);
+        const unsigned nblk = blocks_for(nb, kOrdChunk);
+        hipLaunchKernelGGL(k_ord_blocksum, dim3(nblk), dim3(256), 0, c->stream, nb, c->rdfCnt, c->rdfBlock);
+        hipLaunchKernelGGL(k_ord_scan, dim3(nblk), dim3(256), 0, c->stream, nb, c->rdfCnt, c->rdfBlock);
+        const unsigned gp = blocks_for((long long)p.nMolMe * kRdfLanes, 256);
+        if (c->mixed) {
+            hipLaunchKernelGGL(k_rdfc_scatter<true>, dim3(g), dim3(256), 0, c->stream, p, (const double4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfSlot, c->rdfPos);
+            hipLaunchKernelGGL(k_rdfc_pairs<true>, dim3(gp), dim3(256), shm, c->stream, p, G, (const double4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfPos, c->hist, &c->state->errors);
+        } else {
+            hipLaunchKernelGGL(k_rdfc_scatter<false>, dim3(g), dim3(256), 0, c->stream, p, (const float4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfSlot, c->rdfPos);
+            hipLaunchKernelGGL(k_rdfc_pairs<false>, dim3(gp), dim3(256), shm, c->stream, p, G, (const float4 *)c->pos,
+                               c->rdfCnt, c->rdfKey, c->rdfPos, c->hist, &c->state->errors);
+        }
+        profile_end(c, "rdf");
+        return;
+    }
     if (c->mixed)
         hipLaunchKernelGGL(k_rdf<true>, dim3(g), dim3(256), shm, c->stream, c->dp, (const double4 *)c->pos, cell_bufs(c),
                            c->state, c->hist);

@@ -77,6 +77,14 @@ struct DevState {
     int nMolDeposited;     // metal atoms deposited so far (regime 1)
 };
 
+// Grid of the complete-range RDF: cells at least rangeRdf/2 wide, so a pair within range lies within
+// two cells per axis; s* are the stencil half-widths (the whole axis when 2s+1 would wrap onto itself).
+struct RdfGrid {
+    int nx, ny, nz, ncell;
+    float iwx, iwy, iwz;
+    int sx, sy, sz;
+};
+
 // Device buffers of the centre-of-mass diffusion (sgk_diffusion_enable): nBuff staggered buffers of
 // nVal squared displacements each, in FP64.
 constexpr int kMaxDiffuseBuffers = 256;

@@ -95,3 +95,55 @@ def test_prepared_graph_gives_the_same_run(gpu):
             out.append((recs, r, v))
     assert [x["totEnergy"] for x in out[0][0]] == [x["totEnergy"] for x in out[1][0]]
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+# --------------------------------------------------------------------------------------------
+# complete-range RDF (extension)
+
+def brute_force_rdf(p, r):
+    """Every ordered metal-metal pair within rangeRdf: FP32 displacement with the single-image wrap,
+    r^2 = (dx^2 + dy^2) + dz^2, bin = int(sqrt(r^2) * intervalRdf) - the extension's definition."""
+    n = p.nMolMe
+    x = np.asarray(r[:n, :3], np.float32)
+    L = np.array([p.region.x, p.region.y, p.region.z], np.float32)
+    d = x[:, None, :] - x[None, :, :]
+    half = np.float32(0.5) * L
+    d = np.where(d >= half, d - L, np.where(d < -half, d + L, d)).astype(np.float32)
+    rr = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    np.fill_diagonal(rr, np.inf)
+    range2 = np.float32(np.float32(p.rangeRdf) * np.float32(p.rangeRdf))
+    sel = rr < range2
+    b = (np.sqrt(rr[sel]) * np.float32(p.intervalRdf)).astype(np.int64)
+    return np.bincount(b[b < p.sizeHistRdf], minlength=p.sizeHistRdf).astype(np.int64)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+@pytest.mark.parametrize("name", ["small", "icosa"])
+def test_complete_range_rdf_equals_brute_force(gpu, prec, name):
+    """rdf_complete: the histogram of every metal pair within rangeRdf (10 A for the small workload,
+    its 1.5 rc for the icosahedron) equals the FP32 brute force over all pairs, bin for bin; below one
+    cell width it also equals the reference stencil's histogram (which is complete there)."""
+    s = sg.config_system(name, "config")
+    p = s.params
+    o = oracle.Oracle(p, "f64")
+    err, r, v, a, _ = o.run(s.r, s.v, s.a, 150)
+    assert err == 0
+    rin = r if prec == "mixed" else r.astype(np.float32)
+    hists = {}
+    for complete in (1, 0):
+        print(f"[samples] RDF {name} {prec} complete={complete}", flush=True)
+        with sg.Engine(p, precision=prec, rdf_complete=complete) as eng:
+            eng.upload(rin)
+            eng.bin()
+            eng.build_nebr()
+            eng.rdf_clear()
+            eng.rdf_accumulate()
+            eng.rdf_accumulate()
+            hists[complete] = eng.rdf().astype(np.int64)
+    bf = brute_force_rdf(p, np.asarray(rin, np.float64).astype(np.float32))
+    assert np.array_equal(hists[1], 2 * bf)
+    width = min(p.region.x / p.cells.x, p.region.y / p.cells.y, p.region.z / p.cells.z)
+    inner = int(width * p.intervalRdf)  # bins entirely below one cell width
+    assert inner > 10
+    assert np.array_equal(hists[0][:inner], hists[1][:inner])
+    assert hists[1][inner:].sum() > hists[0][inner:].sum()  # the stencil misses pairs beyond a cell
