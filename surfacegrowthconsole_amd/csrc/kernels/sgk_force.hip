@@ -319,13 +319,15 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
 }
 
 // ---- carbon pass ----------------------------------------------------------------------------
-// Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form:
-// sin(theta - 2pi/3) and cos(theta - 2pi/3) are formed in FP64 (the sine cancels near the
-// equilibrium angle, so it needs the extra precision), the arc from them in FP32 - the deviation is
-// an FP32 quantity of the force law - plus the constant (2pi/3 - theta_0) because theta_0 is the
-// float 2pi/3 of the reference.  Within 45 degrees of 2pi/3 (every bonded angle of a sheet that
-// holds together) the arc is atan of u/v by polynomial, elsewhere atan2f; both carry a relative
-// error of a few ulp.  s = sqrt(1 - c^2) is returned for the 1/sin(theta) factor.
+// Angle deviation acos(c) - theta_0.  Plain form: as the reference, acosf.  Mixed form: the arc of
+// u = sin(theta - 2pi/3) over v = cos(theta - 2pi/3) in FP32 - the deviation is an FP32 quantity
+// of the force law - plus the constant (2pi/3 - theta_0) because theta_0 is the float 2pi/3 of the
+// reference.  u = -s/2 - h c (h = sin(2pi/3), s = sin(theta)) cancels near the equilibrium angle;
+// with c = -1/2 + d and h - s = (h^2 - s^2)/(h + s) = d (d - 1)/(h + s) it becomes
+// u = d ((d - 1)/(2 (h + s)) - h), which does not.  Only d = c + 1/2 is formed in FP64; the rest,
+// s included, is FP32 at full relative precision.  Within 45 degrees of 2pi/3 (every bonded angle of
+// a sheet that holds together) the arc is atan of u/v by polynomial, elsewhere atan2f; both carry a
+// relative error of a few ulp.  s is returned for the 1/sin(theta) factor.
 template <class R>
 struct AngleDev;
 template <>
@@ -338,14 +340,14 @@ struct AngleDev<float> {
 template <>
 struct AngleDev<double> {
     __device__ static __forceinline__ float eval(double c, const DevParams &p, float &s) {
-        const double sd = sqrt64(1.0 - c * c);
-        const double h = 0.86602540378443864676;  // sin(2pi/3)
-        const double u = -0.5 * sd - h * c;        // sin(theta - 2pi/3)
-        const double v = -0.5 * c + h * sd;        // cos(theta - 2pi/3)
-        s = (float)sd;
-        const float fu = (float)u, fv = (float)v;
+        const float h = 0.866025403784f;           // sin(2pi/3)
+        const float d = (float)(c + 0.5);          // c - cos(2pi/3), from the FP64 cosine
+        const float fc = (float)c;
+        s = sqrtf(fmaxf(1.f - fc * fc, 0.f));
+        const float fu = d * (0.5f * (d - 1.f) * rcpf(h + s) - h);  // sin(theta - 2pi/3)
+        const float fv = -0.5f * fc + h * s;                         // cos(theta - 2pi/3)
         const float a = (fv > 0.f && fabsf(fu) <= fv) ? atan_unit(fu * rcpf(fv)) : atan2f(fu, fv);
-        return (float)((double)a + (2.0943951023931954923 - (double)p.theta_0));
+        return a + (float)(2.0943951023931954923 - (double)p.theta_0);
     }
 };
 
