@@ -152,9 +152,10 @@ def batch_workload(nrep=64, seed=2024):
     return out
 
 
-def bench_batch(steps, warmup, nrep):
-    """Atom-steps/s of the whole ensemble batched (one launch per stage) and, for comparison, of
-    the same replicas advanced one after another, each in its own context."""
+def bench_batch(steps, warmup, nrep, tune=True, sequential=True):
+    """Atom-steps/s of the whole ensemble batched (one launch per stage), with lanes per metal atom
+    tuned per replica-size bucket after the warm-up (the table is reported), and, for comparison,
+    of the same replicas advanced one after another, each in its own context."""
     import surfacegrowthconsole_amd as sg
 
     systems = batch_workload(nrep)
@@ -168,6 +169,12 @@ def bench_batch(steps, warmup, nrep):
                 bt.upload(k, s.r, s.v, s.a)
                 bt.set_run_state(k, 0, 1, 0.0)
             bt.run(warmup, records=False)
+            if tune:
+                table = bt.autotune(10)
+                res[f"tune_{prec}"] = [{"metal_atoms": [e[0], e[1]], "replicas": e[2], "atoms": e[3],
+                                        "lanes_per_metal_atom": e[4], "us": round(e[5], 2)} for e in table]
+                res[f"lanes_{prec}"] = bt.replica_lanes()
+                bt.run(warmup, records=False)  # capture the tuned step graphs
             t0 = time.perf_counter()
             done = 0
             while done < steps:
@@ -177,6 +184,8 @@ def bench_batch(steps, warmup, nrep):
             wall = time.perf_counter() - t0
         res[f"batched_{prec}"] = natoms * steps / wall
         log(f"[bench] batch {prec}: {wall / steps * 1e3:.3f} ms/step, {natoms * steps / wall:.4e} atom-steps/s")
+        if not sequential:
+            continue
         # one context per replica, advanced in turn
         engines = []
         for s in systems:
@@ -442,6 +451,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="NREP",
                     help="time an ensemble of NREP uneven replicas, batched and one context each")
     ap.add_argument("--nve-oracle-steps", type=int, default=1000)
+    ap.add_argument("--batch-only", action="store_true", help="with --batch: skip the one-context-per-replica legs")
+    ap.add_argument("--no-tune", action="store_true", help="with --batch: keep the default lanes per metal atom")
     ap.add_argument("--adjoint", action="store_true",
                     help="differentiation workload: adjoint gradients against the FP64 tangent and FD")
     ap.add_argument("--adjoint-equil", type=int, default=2000)
@@ -459,7 +470,8 @@ def main():
         return
     if args.batch:
         build_if_needed()
-        print(json.dumps({"batch": bench_batch(args.steps, args.warmup, args.batch)}))
+        print(json.dumps({"batch": bench_batch(args.steps, args.warmup, args.batch, tune=not args.no_tune,
+                                               sequential=not args.batch_only)}))
         return
     if args.nve:
         build_if_needed()

@@ -270,7 +270,8 @@ int sgk_adhesion_energy(sgk_ctx *ctx, double *e);
  * creation); geometry, atom counts and temperature are free.  Each replica keeps its own cells,
  * lists, step state, reductions and records, and its results equal those of the same replica run
  * alone in an sgk_ctx with xcd_map = 0 and the same lanes per metal atom.  The list build is the
- * LDS-staged one for nl_scheme 2 and the per-atom one otherwise (the lists are the same). */
+ * cell-ordered column walk for nl_scheme 3 (the default; one counting sort orders every replica),
+ * the LDS-staged one for nl_scheme 2 and the per-atom one otherwise (the lists are the same). */
 typedef struct sgk_batch sgk_batch;
 int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfg, sgk_batch **out);
 int sgk_batch_destroy(sgk_batch *b);
@@ -286,7 +287,26 @@ double sgk_batch_last_run_ms(const sgk_batch *b);
 int sgk_batch_rdf_get(sgk_batch *b, int rep, unsigned int *hist);
 int sgk_batch_rdf_clear(sgk_batch *b);
 long long sgk_batch_device_bytes(const sgk_batch *b);
+/* Lanes per metal atom: the common count of every replica (0 when they differ), or of one replica.
+ * At creation every replica gets sgk_config.metal_lanes when it is 1, 2, 4, 8 or 16, else the count
+ * the single-system engine would choose for its size. */
 int sgk_batch_metal_lanes(const sgk_batch *b);
+int sgk_batch_replica_lanes(const sgk_batch *b, int rep);
+/* Set the lanes per metal atom of every replica (lanes[nrep], each 1, 2, 4, 8 or 16).  A replica's
+ * results equal those of the replica run alone in an sgk_ctx with the same lanes. */
+int sgk_batch_set_lanes(sgk_batch *b, const int *lanes);
+/* Autotuner by replica size: replicas are bucketed by metal atoms ([0,128), [128,512), [512,2048),
+ * [2048,8192), [8192,...)); for each bucket every lane count is timed on the live state (density and
+ * force passes of the whole batch, `reps` times) and the fastest kept for the bucket's replicas.
+ * Accelerations and energies are left as the chosen lanes compute them.  Needs one run first.
+ * One entry per (bucket, lane count) tried.  No counterpart in the reference. */
+typedef struct sgk_batch_tune_entry {
+    int metal_lo, metal_hi;  /* bucket: replicas with metal_lo <= nMolMe < metal_hi */
+    int replicas, atoms;     /* replicas and atoms in the bucket */
+    int metal_lanes;         /* lanes per metal atom tried */
+    double us;               /* density + force passes of the batch, microseconds per launch pair */
+} sgk_batch_tune_entry;
+int sgk_batch_autotune(sgk_batch *b, int reps, sgk_batch_tune_entry *entries, int maxEntries, int *nEntries);
 int sgk_batch_atoms(const sgk_batch *b);
 
 /* ---- force evaluation without allocation ---------------------------------------------------------

@@ -102,6 +102,11 @@ class SgkTuneEntry(C.Structure):
     _fields_ = [("metal_lanes", C.c_int), ("xcd_map", C.c_int), ("us", C.c_double)]
 
 
+class SgkBatchTuneEntry(C.Structure):
+    _fields_ = [("metal_lo", C.c_int), ("metal_hi", C.c_int), ("replicas", C.c_int), ("atoms", C.c_int),
+                ("metal_lanes", C.c_int), ("us", C.c_double)]
+
+
 class SgkConfig(C.Structure):
     _fields_ = [
         ("precision", _i), ("nl_capacity", _i), ("cell_capacity", _i), ("lj_capacity", _i),
@@ -165,7 +170,7 @@ EXPORTED_SYMBOLS = [
     "sgk_batch_create", "sgk_batch_destroy", "sgk_batch_last_error", "sgk_batch_upload_f64",
     "sgk_batch_download_f64", "sgk_batch_set_run_state", "sgk_batch_run", "sgk_batch_last_run_ms",
     "sgk_batch_rdf_get", "sgk_batch_rdf_clear", "sgk_batch_device_bytes", "sgk_batch_metal_lanes",
-    "sgk_batch_atoms",
+    "sgk_batch_atoms", "sgk_batch_replica_lanes", "sgk_batch_set_lanes", "sgk_batch_autotune",
     "sgk_adjoint_forward", "sgk_adjoint_backward", "sgk_adjoint_get_info", "sgk_force_vjp",
     "sgk_adhesion_energy", "sgk_prepare_run", "sgk_diffusion_enable", "sgk_diffusion_get", "sgk_forces_noalloc", "sgk_noalloc_scratch_bytes", "sgk_noalloc_last_error",
 ]
@@ -251,6 +256,9 @@ def _proto(lib):
         "sgk_batch_device_bytes": (C.c_longlong, [vp]),
         "sgk_batch_metal_lanes": (_i, [vp]),
         "sgk_batch_atoms": (_i, [vp]),
+        "sgk_batch_replica_lanes": (_i, [vp, _i]),
+        "sgk_batch_set_lanes": (_i, [vp, vp]),
+        "sgk_batch_autotune": (_i, [vp, _i, P(SgkBatchTuneEntry), _i, P(_i)]),
         "sgk_adjoint_forward": (_i, [vp, _i, _i, vp, vp]),
         "sgk_adjoint_backward": (_i, [vp, vp, vp, vp, vp]),
         "sgk_adjoint_get_info": (_i, [vp, P(SgkAdjointInfo)]),

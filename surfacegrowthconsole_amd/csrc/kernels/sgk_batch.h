@@ -23,6 +23,8 @@ struct BatchRep {
     int nbM;             // metal workgroups of the density and force passes
     int part2Off;        // first chunk total of the two-stage step reduction
     int nChunks;         // chunks of kPartChunk force-pass partials
+    int nbNlM;           // workgroups of the metal part of the cell-ordered list build (4 lanes per atom)
+    int ordOff;          // first bucket of the replica's list-build order (2 ncell + 2 buckets)
     int pad;
 };
 
@@ -31,6 +33,8 @@ struct BatchArgs {
     const int2 *atomTab;   // (replica, block) per workgroup over each replica's atoms
     const int2 *forceTab;  // (replica, block) over each replica's nbM + nbC force/density blocks
     const int2 *metalTab;  // (replica, block) over each replica's metal atoms
+    const int2 *nlTab;     // (replica, block) over each replica's cell-ordered list-build blocks
+    const int *order;      // list-build order: replica r's atoms (local indices) at [atomOff, atomOff + nMol)
     void *pos, *vel, *acc;
     float *rho, *embF, *embDF, *fric;
     int *cellOf[2], *cnt[2], *slots[2], *slotOf;

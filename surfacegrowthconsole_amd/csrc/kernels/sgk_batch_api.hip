@@ -113,6 +113,68 @@ void enqueue_batch(sgk_batch *b, const std::string &flags) {
     }
 }
 
+// Lanes per metal atom of a replica with nMe metal atoms when none is configured (the single-system
+// rule measured on MI355X: 4 lanes from 65,536 metal atoms, 8 from 2,048, 16 below).
+int default_lanes(int nMe) { return nMe >= 65536 ? 4 : nMe >= 2048 ? 8 : 16; }
+
+// Force-pass layout for the current lanes per replica: each replica's metal and carbon blocks (as
+// sgk_ctx lays them out), its partials and chunk totals, and the block table grouped by lane count.
+std::vector<int2> force_layout(sgk_batch *b) {
+    std::vector<std::vector<int2>> groups(sgk_batch::kLaneKinds);
+    int partOff = 0;
+    for (int r = 0; r < b->nrep; ++r) {
+        BatchRep &R = b->reps[r];
+        const int G = b->repLanes[r];
+        R.nbM = (int)((((long long)R.p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+        R.nbForce = R.nbM + (R.p.nC + 255) / 256;
+        R.partOff = partOff;
+        R.nChunks = (R.nbForce + 63) / 64;  // sgk::kPartChunk partials per chunk
+        partOff += R.nbForce;
+        auto &g = groups[sgk::batch_lane_index(G)];
+        for (int k = 0; k < R.nbForce; ++k) g.push_back(make_int2(r, k));
+    }
+    for (BatchRep &R : b->reps) {  // chunk totals after all partials
+        R.part2Off = partOff;
+        partOff += R.nChunks;
+    }
+    std::vector<int2> tab;
+    for (int q = 0; q < sgk_batch::kLaneKinds; ++q) {
+        b->grpOff[q] = (int)tab.size();
+        b->grpN[q] = (int)groups[q].size();
+        tab.insert(tab.end(), groups[q].begin(), groups[q].end());
+    }
+    b->nForceBlocks = (int)tab.size();
+    int common = b->repLanes.empty() ? 0 : b->repLanes[0];
+    for (int G : b->repLanes)
+        if (G != common) common = 0;
+    b->metalLanes = common;
+    return tab;
+}
+
+// Largest force-pass layout any lanes give (16 lanes per metal atom): sizes the block table and the
+// partials once.
+void worst_layout(const sgk_batch *b, size_t &tab, size_t &parts) {
+    tab = parts = 0;
+    for (const BatchRep &R : b->reps) {
+        const size_t nb = (size_t)((((long long)R.p.nMolMe * 16 + 255) / 256 + 7) / 8 * 8) + (R.p.nC + 255) / 256;
+        tab += nb;
+        parts += nb + (nb + 63) / 64;
+    }
+}
+
+// Lay out and upload the force pass for the current lanes (tables, replica table); drops the graphs,
+// whose launches carry the old grids.
+int apply_layout(sgk_batch *b) {
+    const std::vector<int2> tab = force_layout(b);
+    BHIP(b, hipMemcpyAsync(b->forceTab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->dReps, b->reps.data(), sizeof(BatchRep) * b->nrep, hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (auto &g : b->graphs)
+        if (g.second.exec) BHIP(b, hipGraphExecDestroy(g.second.exec));
+    b->graphs.clear();
+    return SGK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -186,17 +248,14 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         delete b;
         return bfail(nullptr, SGK_E_HIP, "hipStreamCreate failed");
     }
-    int G = cfg.metal_lanes;
-    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) {
-        const long long avg = sumMe / nrep;
-        G = avg >= 65536 ? 4 : avg >= 2048 ? 8 : 16;
-    }
-    b->metalLanes = G;
-    b->cfg.metal_lanes = G;
+    // lanes per metal atom: the configured count for every replica, else per replica by its size
+    const int G = cfg.metal_lanes;
+    const bool fixedG = G == 1 || G == 2 || G == 4 || G == 8 || G == 16;
+    for (int r = 0; r < nrep; ++r) b->repLanes.push_back(fixedG ? G : default_lanes(params[r].nMolMe));
+    (void)sumMe;
     // replica table and block tables; per-replica workgroup decomposition as in sgk_ctx
-    std::vector<int2> atomTab, forceTab, metalTab;
-    long long atomOff = 0, metalOff = 0, carbonOff = 0, cellOff = 0;
-    int partOff = 0;
+    std::vector<int2> atomTab, metalTab, nlTab;
+    long long atomOff = 0, metalOff = 0, carbonOff = 0, cellOff = 0, ordOff = 0;
     for (int r = 0; r < nrep; ++r) {
         const SimParams &s = params[r];
         BatchRep R;
@@ -210,38 +269,50 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         R.cellOff = cellOff;
         const int nbAtom = (s.nMol + 255) / 256;
         const int nbMetal = (s.nMolMe + 255) / 256;
-        R.nbM = (int)((((long long)s.nMolMe * G + 255) / 256 + 7) / 8 * 8);
         const int nbC = (R.p.nC + 255) / 256;
-        R.nbForce = R.nbM + nbC;
-        R.partOff = partOff;
-        R.nChunks = (R.nbForce + 63) / 64;  // sgk::kPartChunk partials per chunk
         for (int k = 0; k < nbAtom; ++k) atomTab.push_back(make_int2(r, k));
-        for (int k = 0; k < R.nbForce; ++k) forceTab.push_back(make_int2(r, k));
         for (int k = 0; k < nbMetal; ++k) metalTab.push_back(make_int2(r, k));
+        // cell-ordered list build: kNlMetalLanes (4) lanes per metal atom, then one lane per carbon
+        R.nbNlM = (int)(((long long)s.nMolMe * 4 + 255) / 256);
+        const int nbNl = R.nbNlM + nbC;
+        for (int k = 0; k < nbNl; ++k) nlTab.push_back(make_int2(r, k));
+        R.ordOff = (int)ordOff;
+        ordOff += 2LL * R.p.ncell + 2;
         b->reps.push_back(R);
         atomOff += s.nMol;
         metalOff += s.nMolMe;
         carbonOff += R.p.nC;
         cellOff += R.p.ncell;
-        partOff += R.nbForce;
     }
-    for (sgk::BatchRep &R : b->reps) {  // chunk totals after all partials
-        R.part2Off = partOff;
-        partOff += R.nChunks;
-    }
+    const std::vector<int2> forceTab = force_layout(b);
+    size_t tabCap = 0, partCap = 0;
+    worst_layout(b, tabCap, partCap);
     b->nAtoms = atomOff;
     b->nMetal = metalOff;
     b->nCarbon = carbonOff;
     b->nCells = cellOff;
     b->nAtomBlocks = (int)atomTab.size();
-    b->nForceBlocks = (int)forceTab.size();
     b->nMetalBlocks = (int)metalTab.size();
+    b->nNlBlocks = (int)nlTab.size();
+    b->nOrdBuckets = ordOff;
+    if (ordOff >= (1LL << 31)) {
+        delete b;
+        return bfail(nullptr, SGK_E_UNSUPPORTED, "too many cells over all replicas for the list-build order");
+    }
     const int cap = cfg.cell_capacity;
     const size_t Rb = b->mixed ? 8 : 4;
     BTRY(balloc(b, &b->dReps, (size_t)nrep, false));
     BTRY(balloc(b, &b->atomTab, atomTab.size(), false));
-    BTRY(balloc(b, &b->forceTab, forceTab.size(), false));
+    BTRY(balloc(b, &b->forceTab, tabCap, false));
     BTRY(balloc(b, &b->metalTab, metalTab.size(), false));
+    if (cfg.nl_scheme == 3) {
+        BTRY(balloc(b, &b->nlTab, nlTab.size(), false));
+        BTRY(balloc(b, &b->order, (size_t)atomOff, false));
+        BTRY(balloc(b, &b->ordCnt, (size_t)ordOff, false));
+        BTRY(balloc(b, &b->ordKey, (size_t)atomOff, false));
+        BTRY(balloc(b, &b->ordSlot, (size_t)atomOff, false));
+        BTRY(balloc(b, &b->ordBlock, (size_t)(ordOff + 4095) / 4096, false));
+    }
     BTRY(balloc(b, (char **)&b->pos, (size_t)atomOff * (b->mixed ? 32 : 16), true));
     BTRY(balloc(b, (char **)&b->vel, (size_t)atomOff * 3 * Rb, true));
     BTRY(balloc(b, (char **)&b->acc, (size_t)atomOff * 3 * Rb, true));
@@ -267,7 +338,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
     BTRY(balloc(b, &b->ncl, (size_t)carbonOff, true));
     BTRY(balloc(b, &b->bondC, (size_t)carbonOff, true));
     BTRY(balloc(b, &b->bondA, (size_t)carbonOff, true));
-    BTRY(balloc(b, (char **)&b->partials, (size_t)partOff * sgk::partial_bytes_for(b->mixed), true));
+    BTRY(balloc(b, (char **)&b->partials, partCap * sgk::partial_bytes_for(b->mixed), true));
     BTRY(balloc(b, &b->state, (size_t)nrep, true));
     BTRY(balloc(b, &b->records, (size_t)nrep * cfg.records_capacity, true));
     BTRY(balloc(b, &b->hist, (size_t)nrep * s0.sizeHistRdf, true));
@@ -277,6 +348,9 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
              ? SGK_OK : bfail(b, SGK_E_HIP, "copy of the block table"));
     BTRY(hipMemcpyAsync(b->forceTab, forceTab.data(), sizeof(int2) * forceTab.size(), hipMemcpyHostToDevice, b->stream) == hipSuccess
              ? SGK_OK : bfail(b, SGK_E_HIP, "copy of the block table"));
+    if (b->nlTab)
+        BTRY(hipMemcpyAsync(b->nlTab, nlTab.data(), sizeof(int2) * nlTab.size(), hipMemcpyHostToDevice, b->stream) == hipSuccess
+                 ? SGK_OK : bfail(b, SGK_E_HIP, "copy of the block table"));
     if (!metalTab.empty())
         BTRY(hipMemcpyAsync(b->metalTab, metalTab.data(), sizeof(int2) * metalTab.size(), hipMemcpyHostToDevice, b->stream) == hipSuccess
                  ? SGK_OK : bfail(b, SGK_E_HIP, "copy of the block table"));
@@ -384,6 +458,15 @@ int sgk_batch_run(sgk_batch *b, int nsteps, int rdfEvery, sgk_record *records) {
         return bfail(b, SGK_E_ARG, "nsteps exceeds records_capacity; call sgk_batch_run in chunks");
     const std::string flags = batch_flags(b, b->hostStep, nsteps, rdfEvery);
     BHIP(b, hipEventRecord(b->ev0, b->stream));
+    // the list-build order follows the cells of the current positions; refreshed now and then,
+    // outside the graph (the captured launches read it through the same buffer)
+    const bool hadOrder = b->orderValid;
+    if (b->order && (!b->orderValid || b->orderAge >= sgk::kNlOrderSteps)) sgk::build_batch_nl_order(b);
+    if (!hadOrder && b->orderValid) {  // graphs captured before the order existed read none
+        for (auto &g : b->graphs)
+            if (g.second.exec) BHIP(b, hipGraphExecDestroy(g.second.exec));
+        b->graphs.clear();
+    }
     if (b->cfg.use_graphs) {
         auto it = b->graphs.find(flags);
         if (it == b->graphs.end()) {
@@ -421,6 +504,7 @@ int sgk_batch_run(sgk_batch *b, int nsteps, int rdfEvery, sgk_record *records) {
     (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
     b->lastRunMs = ms;
     b->hostStep += nsteps;
+    b->orderAge += nsteps;
     return check_batch_flags(b);
 }
 
@@ -444,6 +528,79 @@ int sgk_batch_rdf_clear(sgk_batch *b) {
 long long sgk_batch_device_bytes(const sgk_batch *b) { return b ? b->bytes : 0; }
 
 int sgk_batch_metal_lanes(const sgk_batch *b) { return b ? b->metalLanes : 0; }
+
+int sgk_batch_replica_lanes(const sgk_batch *b, int rep) {
+    return (b && rep >= 0 && rep < b->nrep) ? b->repLanes[rep] : 0;
+}
+
+int sgk_batch_set_lanes(sgk_batch *b, const int *lanes) {
+    if (!b || !lanes) return SGK_E_ARG;
+    for (int r = 0; r < b->nrep; ++r)
+        if (lanes[r] != 1 && lanes[r] != 2 && lanes[r] != 4 && lanes[r] != 8 && lanes[r] != 16)
+            return bfail(b, SGK_E_ARG, "lanes per metal atom must be 1, 2, 4, 8 or 16");
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->repLanes.assign(lanes, lanes + b->nrep);
+    return apply_layout(b);
+}
+
+int sgk_batch_autotune(sgk_batch *b, int reps, sgk_batch_tune_entry *entries, int maxEntries, int *nEntries) {
+    if (!b) return SGK_E_ARG;
+    if (b->hostStep == 0) return bfail(b, SGK_E_STATE, "no neighbour lists yet: run at least one step first");
+    if (reps <= 0) reps = 20;
+    static const int kBounds[] = {0, 128, 512, 2048, 8192, 1 << 30};
+    static const int kinds[] = {1, 2, 4, 8, 16};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
+    int n = 0, rc = SGK_OK;
+    for (int k = 0; k + 1 < (int)(sizeof(kBounds) / sizeof(int)) && rc == SGK_OK; ++k) {
+        std::vector<int> members;
+        for (int r = 0; r < b->nrep; ++r)
+            if (b->sp[r].nMolMe >= kBounds[k] && b->sp[r].nMolMe < kBounds[k + 1] && b->sp[r].nMolMe > 0)
+                members.push_back(r);
+        if (members.empty()) continue;
+        int bestG = b->repLanes[members[0]];
+        double best = 1e300;
+        for (int G : kinds) {
+            for (int r : members) b->repLanes[r] = G;
+            if ((rc = apply_layout(b)) != SGK_OK) break;
+            sgk::launch_batch_rho(b);  // warm-up
+            sgk::launch_batch_force_eval(b);
+            BHIP(b, hipEventRecord(e0, b->stream));
+            for (int q = 0; q < reps; ++q) {
+                sgk::launch_batch_rho(b);
+                sgk::launch_batch_force_eval(b);
+            }
+            BHIP(b, hipEventRecord(e1, b->stream));
+            BHIP(b, hipGetLastError());
+            BHIP(b, hipEventSynchronize(e1));
+            float ms = 0.f;
+            BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+            const double us = 1e3 * ms / reps;
+            if (entries && n < maxEntries) {
+                long long atoms = 0;
+                for (int r : members) atoms += b->sp[r].nMol;
+                entries[n] = sgk_batch_tune_entry{kBounds[k], kBounds[k + 1], (int)members.size(), (int)atoms, G, us};
+            }
+            ++n;
+            if (us < best) {
+                best = us;
+                bestG = G;
+            }
+        }
+        for (int r : members) b->repLanes[r] = bestG;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SGK_OK) return rc;
+    if ((rc = apply_layout(b)) != SGK_OK) return rc;
+    // leave accelerations, energies and friction as the chosen lanes compute them
+    sgk::launch_batch_rho(b);
+    sgk::launch_batch_force_eval(b);
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (nEntries) *nEntries = n;
+    return check_batch_flags(b);
+}
 
 int sgk_batch_atoms(const sgk_batch *b) { return b ? (int)b->nAtoms : 0; }
 

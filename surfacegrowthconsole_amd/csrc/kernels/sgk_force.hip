@@ -659,24 +659,34 @@ void launch_rho(sgk_ctx *c) {
     profile_end(c, "rho");
 }
 
+int batch_lane_index(int G) { return G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : G == 8 ? 3 : 4; }
+
+// One launch per lane count present: the replicas with G lanes per metal atom, through their own
+// block table (the kernel body is templated on G).
 template <bool M, class T>
 static void batch_dispatch(sgk_batch *b, bool force, int integrate) {
-    const dim3 g(b->nForceBlocks), t(256);
+    static const int kinds[sgk_batch::kLaneKinds] = {1, 2, 4, 8, 16};
+    for (int q = 0; q < sgk_batch::kLaneKinds; ++q) {
+        if (b->grpN[q] == 0) continue;
+        BatchArgs a = b->args();
+        a.forceTab = b->forceTab + b->grpOff[q];
+        const dim3 g(b->grpN[q]), t(256);
 #define SGK_B(GG)                                                                            \
     do {                                                                                     \
         if (force)                                                                           \
-            hipLaunchKernelGGL((kb_force<M, T, GG>), g, t, 0, b->stream, b->args(), integrate); \
+            hipLaunchKernelGGL((kb_force<M, T, GG>), g, t, 0, b->stream, a, integrate);     \
         else                                                                                 \
-            hipLaunchKernelGGL((kb_rho<M, T, GG>), g, t, 0, b->stream, b->args());           \
+            hipLaunchKernelGGL((kb_rho<M, T, GG>), g, t, 0, b->stream, a);                  \
     } while (0)
-    switch (b->metalLanes) {
-        case 1: SGK_B(1); break;
-        case 2: SGK_B(2); break;
-        case 4: SGK_B(4); break;
-        case 16: SGK_B(16); break;
-        default: SGK_B(8); break;
-    }
+        switch (kinds[q]) {
+            case 1: SGK_B(1); break;
+            case 2: SGK_B(2); break;
+            case 4: SGK_B(4); break;
+            case 16: SGK_B(16); break;
+            default: SGK_B(8); break;
+        }
 #undef SGK_B
+    }
 }
 
 static void batch_launch(sgk_batch *b, bool force, int integrate) {
@@ -691,6 +701,7 @@ static void batch_launch(sgk_batch *b, bool force, int integrate) {
 
 void launch_batch_rho(sgk_batch *b) { batch_launch(b, false, 0); }
 void launch_batch_force(sgk_batch *b, bool prekick) { batch_launch(b, true, prekick ? 2 : 1); }
+void launch_batch_force_eval(sgk_batch *b) { batch_launch(b, true, 0); }
 
 void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick) {
     if (c->nbM + c->nbC == 0) return;

@@ -93,11 +93,20 @@ struct sgk_batch {
     bool mixed = false;
     int device = 0;
     hipStream_t stream = nullptr;
-    int metalLanes = 4;
+    int metalLanes = 4;                 // the common lanes per metal atom (0 when replicas differ)
+    std::vector<int> repLanes;          // lanes per metal atom of each replica
+    // density and force passes: one launch per distinct lane count, its blocks at forceTab[grpOff[g]]
+    static constexpr int kLaneKinds = 5;  // 1, 2, 4, 8, 16
+    int grpOff[kLaneKinds] = {}, grpN[kLaneKinds] = {};
     std::vector<sgk::BatchRep> reps;  // host copy of the replica table
     sgk::BatchRep *dReps = nullptr;
-    int2 *atomTab = nullptr, *forceTab = nullptr, *metalTab = nullptr;
-    int nAtomBlocks = 0, nForceBlocks = 0, nMetalBlocks = 0;
+    int2 *atomTab = nullptr, *forceTab = nullptr, *metalTab = nullptr, *nlTab = nullptr;
+    int nAtomBlocks = 0, nForceBlocks = 0, nMetalBlocks = 0, nNlBlocks = 0;
+    // cell-ordered list build (nl_scheme 3): the order of every replica from one counting sort
+    int *order = nullptr, *ordCnt = nullptr, *ordKey = nullptr, *ordSlot = nullptr, *ordBlock = nullptr;
+    long long nOrdBuckets = 0;
+    bool orderValid = false;
+    int orderAge = 0;
     long long nAtoms = 0, nMetal = 0, nCarbon = 0, nCells = 0;
     void *pos = nullptr, *vel = nullptr, *acc = nullptr;
     float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;
@@ -126,6 +135,8 @@ struct sgk_batch {
         a.atomTab = atomTab;
         a.forceTab = forceTab;
         a.metalTab = metalTab;
+        a.nlTab = nlTab;
+        a.order = orderValid ? order : nullptr;
         a.pos = pos;
         a.vel = vel;
         a.acc = acc;
@@ -170,6 +181,7 @@ int prepare_ctx(const SimParams *sp, const sgk_config *cfg, sgk_ctx *c, std::str
 void launch_bin(sgk_ctx *c, bool force);
 void launch_batch_sortcells(sgk_batch *b);
 void launch_batch_nl(sgk_batch *b);
+void build_batch_nl_order(sgk_batch *b);
 void launch_batch_rdf(sgk_batch *b);
 void launch_sortcells(sgk_ctx *c, bool force);
 void launch_nl(sgk_ctx *c, bool force);
@@ -181,7 +193,9 @@ RdfGrid rdf_grid(const DevParams &p);  // grid of the complete-range RDF (ncell 
 // sgk_force.hip
 void launch_rho(sgk_ctx *c);
 void launch_batch_rho(sgk_batch *b);
+int batch_lane_index(int G);  // position of G in the lane kinds 1, 2, 4, 8, 16
 void launch_batch_force(sgk_batch *b, bool prekick = false);
+void launch_batch_force_eval(sgk_batch *b);  // forces, energies and partials only: velocities untouched
 // prekick: the force pass also applies the next step's first half-kick (the caller guarantees that
 // nothing touches velocities or accelerations before that step's launch_lf1(kicked = true)).
 void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick = false);

@@ -854,22 +854,24 @@ __global__ __launch_bounds__(256) void k_nlc(DevParams p, int nbM, const typenam
 // atoms move; it decides which lanes build which atom's lists, never the lists.
 constexpr int kOrdChunk = 4096;  // buckets per scan block, 16 per thread
 
+// Bucket of atom n in the list-build order: its column-major cell, carbons first.
+__device__ __forceinline__ int ord_bucket(const DevParams &p, const float3 q, int n) {
+#pragma clang fp contract(off)
+    const float sx = q.x + 0.5f * p.Lx, sy = q.y + 0.5f * p.Ly, sz = q.z + 0.5f * p.Lz;
+    const int cx = (int)(sx * p.iwx), cy = (int)(sy * p.iwy), cz = (int)(sz * p.iwz);
+    const int c = (cy * p.cx + cx) * p.cz + cz;  // column-major: a column's z cells are adjacent
+    int k = (c < 0 || c >= p.ncell) ? p.ncell : c;
+    if (n < p.nMolMe) k += p.ncell + 1;  // metal buckets after the carbon ones
+    return k;
+}
+
 template <bool M>
 __global__ __launch_bounds__(256) void k_ord_count(DevParams p, const typename Prec<M>::P4 *__restrict__ pos,
                                                    int *__restrict__ cnt, int *__restrict__ key,
                                                    int *__restrict__ slot) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
-    int k;
-    {
-#pragma clang fp contract(off)
-        const float3 q = f3(pos[n]);
-        const float sx = q.x + 0.5f * p.Lx, sy = q.y + 0.5f * p.Ly, sz = q.z + 0.5f * p.Lz;
-        const int cx = (int)(sx * p.iwx), cy = (int)(sy * p.iwy), cz = (int)(sz * p.iwz);
-        const int c = (cy * p.cx + cx) * p.cz + cz;  // column-major: a column's z cells are adjacent
-        k = (c < 0 || c >= p.ncell) ? p.ncell : c;
-        if (n < p.nMolMe) k += p.ncell + 1;  // metal buckets after the carbon ones
-    }
+    const int k = ord_bucket(p, f3(pos[n]), n);
     // lanes with the same bucket share one atomicAdd, as in binning
     const int lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -965,6 +967,52 @@ __global__ __launch_bounds__(256) void kb_nlt(BatchArgs a) {
     nlt_body<M>(p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0, 1,
                 a.nlM + B.metalOff * p.nlCap, a.nnM + B.metalOff, a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff,
                 a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
+}
+
+// Batched list-build order: one counting sort over every replica, replica r's buckets at
+// [ordOff, ordOff + 2 ncell + 2), so its atoms land at [atomOff, atomOff + nMol) of the order, by
+// local index.  The slot within a bucket comes from a plain atomicAdd: it decides only which lane
+// builds which atom's lists.
+template <bool M>
+__global__ __launch_bounds__(256) void kb_ord_count(BatchArgs a, int *__restrict__ cnt, int *__restrict__ key,
+                                                    int *__restrict__ slot) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const int n = e.y * blockDim.x + threadIdx.x;
+    if (n >= B.p.nMol) return;
+    const int k = B.ordOff + ord_bucket(B.p, f3(((const typename Prec<M>::P4 *)a.pos)[B.atomOff + n]), n);
+    key[B.atomOff + n] = k;
+    slot[B.atomOff + n] = atomicAdd(&cnt[k], 1);
+}
+
+__global__ __launch_bounds__(256) void kb_ord_scatter(BatchArgs a, const int *__restrict__ start,
+                                                      const int *__restrict__ key, const int *__restrict__ slot,
+                                                      int *__restrict__ order) {
+    const int2 e = a.atomTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const int n = e.y * blockDim.x + threadIdx.x;
+    if (n >= B.p.nMol) return;
+    order[start[key[B.atomOff + n]] + slot[B.atomOff + n]] = n;
+}
+
+// Batched cell-ordered list build (the step path: from the new cell buffer before the commit, only
+// replicas that rebuild this step).
+template <bool M>
+__global__ __launch_bounds__(256) void kb_nlc(BatchArgs a) {
+    const int2 e = a.nlTab[blockIdx.x];
+    const BatchRep &B = a.reps[e.x];
+    const DevParams &p = B.p;
+    DevState *st = a.state + e.x;
+    if (!st->nebrNow) return;
+    const int b = st->cellbuf ^ 1;
+    const typename Prec<M>::P4 *pos = (const typename Prec<M>::P4 *)a.pos + B.atomOff;
+    const int *order = a.order ? a.order + B.atomOff : nullptr;
+    if (e.y < B.nbNlM)
+        nlm_body<M, kNlMetalLanes, false>(p, e.y, pos, rep_cells(a, B), b, st, order, a.nlM + B.metalOff * p.nlCap,
+                                          a.nnM + B.metalOff, nullptr, nullptr);
+    else
+        nlc_body<M, false>(p, e.y - B.nbNlM, pos, rep_cells(a, B), b, st, order, a.lj + B.carbonOff * p.ljCap,
+                           a.nlj + B.carbonOff, a.cl + B.carbonOff * p.clCap, a.ncl + B.carbonOff, nullptr, nullptr);
 }
 
 // Make the freshly built buffer current (API path; the step path does this in its reduction).
@@ -1251,7 +1299,34 @@ void launch_batch_sortcells(sgk_batch *b) {
         hipLaunchKernelGGL(kb_sortcells<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args());
 }
 
+void build_batch_nl_order(sgk_batch *b) {
+    if (!b->order || b->nAtoms == 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min(blocks_for(b->nOrdBuckets, 256), 4096u)), dim3(256), 0, b->stream,
+                       b->ordCnt, b->nOrdBuckets, 0);
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_ord_count<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), b->ordCnt,
+                           b->ordKey, b->ordSlot);
+    else
+        hipLaunchKernelGGL(kb_ord_count<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), b->ordCnt,
+                           b->ordKey, b->ordSlot);
+    const int nb = (int)b->nOrdBuckets;
+    const unsigned nblk = blocks_for(nb, kOrdChunk);
+    hipLaunchKernelGGL(k_ord_blocksum, dim3(nblk), dim3(256), 0, b->stream, nb, b->ordCnt, b->ordBlock);
+    hipLaunchKernelGGL(k_ord_scan, dim3(nblk), dim3(256), 0, b->stream, nb, b->ordCnt, b->ordBlock);
+    hipLaunchKernelGGL(kb_ord_scatter, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), b->ordCnt, b->ordKey,
+                       b->ordSlot, b->order);
+    b->orderValid = true;
+    b->orderAge = 0;
+}
+
 void launch_batch_nl(sgk_batch *b) {
+    if (b->cfg.nl_scheme == 3 && b->nlTab) {
+        if (b->mixed)
+            hipLaunchKernelGGL(kb_nlc<true>, dim3(b->nNlBlocks), dim3(256), 0, b->stream, b->args());
+        else
+            hipLaunchKernelGGL(kb_nlc<false>, dim3(b->nNlBlocks), dim3(256), 0, b->stream, b->args());
+        return;
+    }
     const bool tiled = b->cfg.nl_scheme == 2;
     if (b->mixed) {
         if (tiled)

@@ -384,6 +384,23 @@ class Batch:
     def atoms(self):
         return self.lib.sgk_batch_atoms(self.h)
 
+    def replica_lanes(self):
+        """Lanes per metal atom of every replica."""
+        return [self.lib.sgk_batch_replica_lanes(self.h, r) for r in range(self.nrep)]
+
+    def set_lanes(self, lanes):
+        arr = np.ascontiguousarray(lanes, np.int32)
+        assert arr.shape == (self.nrep,)
+        self._check(self.lib.sgk_batch_set_lanes(self.h, _ptr(arr)), "sgk_batch_set_lanes")
+
+    def autotune(self, reps=20):
+        """Per replica-size bucket, the fastest lanes per metal atom on the current state; returns
+        [(metal_lo, metal_hi, replicas, atoms, lanes, us), ...] for every (bucket, lanes) tried."""
+        buf = (N.SgkBatchTuneEntry * 64)()
+        n = C.c_int()
+        self._check(self.lib.sgk_batch_autotune(self.h, reps, buf, 64, C.byref(n)), "sgk_batch_autotune")
+        return [(e.metal_lo, e.metal_hi, e.replicas, e.atoms, e.metal_lanes, e.us) for e in buf[: n.value]]
+
     def upload(self, rep, r, v=None, a=None):
         n = self.params[rep].nMol
         r4 = np.zeros((n, 4))

@@ -58,3 +58,60 @@ def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
         assert np.array_equal(out[k][1], v), k
         assert [x["totEnergy"] for x in recs[k]] == [x["totEnergy"] for x in ref], k
         assert [x["rebuilt"] for x in recs[k]] == [x["rebuilt"] for x in ref], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_batch_with_lanes_per_replica_equals_replicas_run_alone(gpu, prec):
+    """Replicas with different lanes per metal atom in one batch (one density/force launch per lane
+    count): each equals the replica run alone with its own lanes, bit for bit."""
+    systems = replicas()
+    lanes = [16, 2, 8, 4]
+    steps = 30
+    print(f"[batch] {prec}: lanes per replica {lanes}", flush=True)
+    with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, xcd_map=0) as bt:
+        bt.set_lanes(lanes)
+        assert bt.replica_lanes() == lanes and bt.metal_lanes == 0
+        for k, s in enumerate(systems):
+            bt.upload(k, s.r, s.v, s.a)
+            bt.set_run_state(k, 0, 1, 0.0)
+        recs = bt.run(steps)
+        out = [bt.download(k) for k in range(len(systems))]
+    for k, s in enumerate(systems):
+        with sg.Engine(s.params, precision=prec, rebuild_every=10, metal_lanes=lanes[k], xcd_map=0) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            ref = eng.run(steps)
+            r, v, a = eng.download()
+        assert np.array_equal(out[k][0][:, :3], r[:, :3]), k
+        assert np.array_equal(out[k][1], v), k
+        assert [x["totEnergy"] for x in recs[k]] == [x["totEnergy"] for x in ref], k
+
+
+@pytest.mark.gpu
+def test_batch_autotune_per_size_bucket(gpu):
+    """The autotuner tries every lane count for each replica-size bucket and keeps the fastest; the
+    forces it leaves equal the untuned ones to FP64 summation order (the lanes change only the order
+    in which a metal atom's pair terms are added)."""
+    systems = replicas()
+    with sg.Batch([s.params for s in systems], precision="mixed", rebuild_every=10, xcd_map=0) as bt:
+        for k, s in enumerate(systems):
+            bt.upload(k, s.r, s.v, s.a)
+            bt.set_run_state(k, 0, 1, 0.0)
+        bt.run(10, records=False)
+        before = [bt.download(k)[2] for k in range(len(systems))]
+        table = bt.autotune(5)
+        after = [bt.download(k)[2] for k in range(len(systems))]
+        lanes = bt.replica_lanes()
+        recs = bt.run(20)
+    buckets = {(e[0], e[1]) for e in table}
+    metal = [s.params.nMolMe for s in systems]
+    assert len(buckets) == len({next(b for b in buckets if b[0] <= m < b[1]) for m in metal})
+    assert len(table) == 5 * len(buckets)  # every lane count per bucket
+    for lo, hi in buckets:  # the chosen lanes are the fastest of the bucket's entries
+        best = min((e for e in table if (e[0], e[1]) == (lo, hi)), key=lambda e: e[5])[4]
+        assert all(lanes[k] == best for k, m in enumerate(metal) if lo <= m < hi)
+    for b, a in zip(before, after):
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-12 * scale
+    assert all(np.isfinite(x["totEnergy"]) and x["errors"] == 0 for rr in recs for x in rr)
