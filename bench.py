@@ -353,7 +353,8 @@ def bench_adjoint(equil, window, every, fd_steps):
     `every` steps.  Gradients of the window-mean adhesion energy and the final cluster height with
     respect to the scale factors of LJ epsilon, LJ sigma, embedding F and density f from the GPU
     adjoint (mixed form), against the tangent build of the FP64 oracle on the same initial state
-    and against central differences of the FP64 oracle at each relative step in fd_steps."""
+    and against central differences of the FP64 oracle at each relative step in fd_steps, free and
+    with the unperturbed run's discrete decisions frozen."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -414,20 +415,32 @@ def bench_adjoint(equil, window, every, fd_steps):
         "oracle_tangent_s": t_tan,
         "fd": {},
     }
+    # Central differences free (each perturbed run takes its own decisions) and frozen (the decision
+    # tape of the unperturbed run replayed: rebuild steps, lists, cutoff bits, F(rho) branches).
+    out["fd_frozen"] = {}
+    if fd_steps:
+        o.freeze("record")
+        o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)
+        o.freeze("off")
     for h in fd_steps:
-        log(f"[adjoint] central differences, h = {h:g}")
-        fd = np.zeros((2, 4))
-        for k in range(4):
-            Js = []
-            for sgn in (1, -1):
-                sc = np.ones(4)
-                sc[k] = 1 + sgn * h
-                o.set_scale(sc)
-                Js.append(o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)[0])
-            fd[:, k] = (Js[0] - Js[1]) / (2 * h)
-        o.set_scale(np.ones(4))
-        out["fd"][f"{h:g}"] = {"grad": fd.tolist(), "rel_err_gpu": (np.abs(grad - fd) / np.abs(fd)).tolist(),
-                               "rel_err_tangent": (np.abs(dJ - fd) / np.abs(fd)).tolist()}
+        for frozen in (False, True):
+            log(f"[adjoint] central differences, h = {h:g}, {'frozen' if frozen else 'free'}")
+            fd = np.zeros((2, 4))
+            for k in range(4):
+                Js = []
+                for sgn in (1, -1):
+                    sc = np.ones(4)
+                    sc[k] = 1 + sgn * h
+                    o.set_scale(sc)
+                    o.freeze("replay" if frozen else "off")
+                    Js.append(o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)[0])
+                    assert not frozen or o.freeze_ok()
+                fd[:, k] = (Js[0] - Js[1]) / (2 * h)
+            o.set_scale(np.ones(4))
+            o.freeze("off")
+            out["fd_frozen" if frozen else "fd"][f"{h:g}"] = {
+                "grad": fd.tolist(), "rel_err_gpu": (np.abs(grad - fd) / np.abs(fd)).tolist(),
+                "rel_err_tangent": (np.abs(dJ - fd) / np.abs(fd)).tolist()}
     return out
 
 

@@ -12,7 +12,10 @@
  * The integer parts (cell index, neighbour list, RDF bins) of *_f32 are bit-exact definitions.
  *
  * tangent.cpp compiles the same source a third time with forward-mode dual numbers (dual.hpp): the
- * FP64 values plus directional derivatives, the reference gradient of the adjoint.
+ * FP64 values plus directional derivatives, the reference gradient of the adjoint.  The FP64 build
+ * also keeps a decision tape (orc_freeze_f64): a run that replays it holds the discrete decisions of
+ * a recorded run, so central differences of perturbed runs are those of the map the adjoint and the
+ * tangent differentiate (pinned against the tangent at a step where free differences fail).
  *
  * Pinning (tests/test_oracle_*.py, tests/test_graphene_pinning.py): EAM by fcc shell sums (closed
  * form), two-atom LJ, central differences of the conservative EAM+LJ energy, the graphene law by the
@@ -171,6 +174,14 @@ int orc_tangent_window(const orc_params *p, double *pos4, double *vel, double *a
  * energy *adh and dadh[ORC_ND]. */
 void orc_tangent_forces(const orc_params *p, const double *pos4, const int *NN, const int *NBL, const double *pdir,
                         const double *rdir, double *acc, double *dacc, double *adh, double *dadh);
+
+/* Decision tape of the FP64 step loop (see oracle_impl.c): mode 1 clears it and records the rebuild
+ * flag of every step, every list built and the F(rho) branch of every metal atom at every force pass
+ * of the runs that follow (orc_run_f64, orc_window_f64); mode 2 replays them from the start instead of
+ * deciding; 0 turns it off.  Returns the tape length (ints). */
+long long orc_freeze_f64(int mode);
+/* 1 when the replay since the last orc_freeze_f64(2) took exactly the recorded decisions. */
+int orc_freeze_ok_f64(void);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,9 @@ def lib():
         _lib.orc_diffusion_create.restype = vp
         _lib.orc_diffusion_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, vp]
         _lib.orc_diffusion_destroy.argtypes = [vp]
+        _lib.orc_freeze_f64.restype = C.c_longlong
+        _lib.orc_freeze_f64.argtypes = [C.c_int]
+        _lib.orc_freeze_ok_f64.restype = C.c_int
         _lib.orc_tangent_window.restype = C.c_int
         _lib.orc_tangent_window.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _lib.orc_tangent_forces.restype = None
@@ -275,6 +278,16 @@ class Oracle:
         J = np.zeros(2, self.dt)
         err = self.f("orc_window")(C.byref(self.p), _p(r), _p(v), _p(a), nsteps, step0, rebuild_every, _p(J))
         return J.astype(np.float64), r, v, a, err
+
+    def freeze(self, mode):
+        """FP64 decision tape: "record" (clear, then record the next runs' rebuild steps, lists and
+        F(rho) branches), "replay" (take them from the tape, from the start) or "off".  Returns the
+        tape length."""
+        return int(self.l.orc_freeze_f64({"off": 0, "record": 1, "replay": 2}[mode]))
+
+    def freeze_ok(self):
+        """True when the replay since freeze("replay") took exactly the recorded decisions."""
+        return bool(self.l.orc_freeze_ok_f64())
 
     def tangent_window(self, r, v, nsteps, step0=0, rebuild_every=0, pdir=None, rdir=None, vdir=None):
         """FP64 window with forward-mode derivatives along up to ND directions: pdir (ND,4) seeds the

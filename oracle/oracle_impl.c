@@ -203,14 +203,22 @@ static inline REAL eam_df(const orc_params *p, REAL r) {
     REAL dB = K(1.) + POW(x - la, K(20.));
     return -PAR_DENS(p) * rei * EXP(-be * (x - K(1.))) * (be + K(20.) * POW(x - la, K(19.)) / dB) / dB;
 }
-static inline REAL eam_F(const orc_params *p, REAL rho) {
+/* Branch of the embedding function at density rho: 0 below 0.85 rho_e, 1 up to 1.15 rho_e, 2 above
+ * (the branch points of .cu:72-103 and 160-195). */
+static inline int eam_branch(const orc_params *p, REAL rho) {
     const REAL rhoe = p->rhoe, rhon = K(0.85) * rhoe, rhoo = K(1.15) * rhoe;
-    if (rho < rhon) { /* .cu:72-82 */
+    return rho < rhon ? 0 : (rho < rhoo ? 1 : 2);
+}
+
+/* F(rho) on branch br (br < 0: the branch of rho). */
+static inline REAL eam_F_br(const orc_params *p, REAL rho, int br) {
+    if (br < 0) br = eam_branch(p, rho);
+    if (br == 0) { /* .cu:72-82 */
         REAL rhoni = (REAL)p->rhoei * K(1.176471);
         REAL t = rho * rhoni - K(1.);
         return PAR_FN(p, 0) + PAR_FN(p, 1) * t + PAR_FN(p, 2) * t * t + PAR_FN(p, 3) * t * t * t;
     }
-    if (rho < rhoo) { /* .cu:84-94 */
+    if (br == 1) { /* .cu:84-94 */
         REAL t = rho * (REAL)p->rhoei - K(1.);
         return PAR_F(p, 0) + PAR_F(p, 1) * t + PAR_F(p, 2) * t * t + PAR_F(p, 3) * t * t * t;
     }
@@ -218,14 +226,17 @@ static inline REAL eam_F(const orc_params *p, REAL rho) {
     REAL x = rho * (REAL)p->rhoei;
     return PAR_FE(p) * (K(1.) - LOG(POW(x, (REAL)p->eta))) * POW(x, (REAL)p->eta);
 }
-static inline REAL eam_dF(const orc_params *p, REAL rho) {
-    const REAL rhoe = p->rhoe, rhon = K(0.85) * rhoe, rhoo = K(1.15) * rhoe;
-    if (rho < rhon) { /* .cu:160-171 */
+static inline REAL eam_F(const orc_params *p, REAL rho) { return eam_F_br(p, rho, -1); }
+
+/* F'(rho) on branch br (br < 0: the branch of rho). */
+static inline REAL eam_dF_br(const orc_params *p, REAL rho, int br) {
+    if (br < 0) br = eam_branch(p, rho);
+    if (br == 0) { /* .cu:160-171 */
         REAL rhoni = (REAL)p->rhoei * K(1.176471);
         REAL t = rho * rhoni - K(1.);
         return rhoni * (PAR_FN(p, 1) + K(2.) * PAR_FN(p, 2) * t + K(3.) * PAR_FN(p, 3) * t * t);
     }
-    if (rho < rhoo) { /* .cu:173-184 */
+    if (br == 1) { /* .cu:173-184 */
         REAL x = rho * (REAL)p->rhoei;
         return (REAL)p->rhoei * (PAR_F(p, 1) + K(2.) * PAR_F(p, 2) * (x - K(1.)) +
                                  K(3.) * PAR_F(p, 3) * (x - K(1.)) * (x - K(1.)));
@@ -234,6 +245,57 @@ static inline REAL eam_dF(const orc_params *p, REAL rho) {
     REAL x = rho * (REAL)p->rhoei, eta = p->eta;
     return -PAR_FE(p) * eta * (REAL)p->rhoei * POW(x, (eta - K(1.))) * LOG(POW(x, eta));
 }
+static inline REAL eam_dF(const orc_params *p, REAL rho) { return eam_dF_br(p, rho, -1); }
+
+/* ---- decision tape of the FP64 build (orc_freeze_f64) ------------------------------------------
+ * The adjoint differentiates the step map with its discrete decisions held: the neighbour lists
+ * (contents and order), the steps that rebuild them, the branch of F(rho) of every metal atom and
+ * which list pairs lie inside the EAM and LJ cutoffs.  In record mode the step loop appends each
+ * decision in the order it takes them (the rebuild flag of every step, every list built, the cutoff
+ * bits of every density, force and adhesion pass, the branches of every force pass); in replay mode it takes them from
+ * the tape instead, so a run with perturbed parameters follows the recorded run's decisions and its
+ * finite differences are those of the map the adjoint differentiates.  Test infrastructure only. */
+#if defined(ORC_F64)
+static struct {
+    int mode; /* 0 off, 1 record, 2 replay */
+    int *v;
+    size_t len, cap, pos;
+    int bad; /* a replay ran past the end of the tape */
+} g_tape;
+static inline int tape_mode(void) { return g_tape.mode; }
+static inline void tape_push(const int *x, size_t n) {
+    if (g_tape.len + n > g_tape.cap) {
+        size_t c = g_tape.cap ? 2 * g_tape.cap : 1 << 20;
+        while (c < g_tape.len + n) c *= 2;
+        g_tape.v = (int *)realloc(g_tape.v, c * sizeof(int));
+        g_tape.cap = c;
+    }
+    memcpy(g_tape.v + g_tape.len, x, n * sizeof(int));
+    g_tape.len += n;
+}
+static inline void tape_pull(int *x, size_t n) {
+    if (g_tape.pos + n > g_tape.len) {
+        g_tape.bad = 1;
+        memset(x, 0, n * sizeof(int));
+        return;
+    }
+    memcpy(x, g_tape.v + g_tape.pos, n * sizeof(int));
+    g_tape.pos += n;
+}
+long long orc_freeze_f64(int mode) {
+    if (mode == 1) g_tape.len = 0;
+    g_tape.pos = 0;
+    g_tape.bad = 0;
+    g_tape.mode = mode;
+    return (long long)g_tape.len;
+}
+/* 1 when the last replay took exactly the recorded decisions, 0 otherwise. */
+int orc_freeze_ok_f64(void) { return !g_tape.bad && g_tape.pos == g_tape.len; }
+#else
+static inline int tape_mode(void) { return 0; }
+static inline void tape_push(const int *x, size_t n) { (void)x; (void)n; }
+static inline void tape_pull(int *x, size_t n) { (void)x; (void)n; }
+#endif
 
 REAL FN(orc_embed)(const orc_params *p, REAL rho, REAL *dF) {
     if (dF) *dF = eam_dF(p, rho);
@@ -247,9 +309,52 @@ void FN(orc_pair)(const orc_params *p, REAL r, REAL *out4) {
     out4[3] = eam_df(p, r);
 }
 
+/* Cutoff decisions of one pass over list rows [0, nRows), for the decision tape: entry j of row i is
+ * 1 when the pair lies inside its cutoff (rrCutEam for two metal atoms, rrCutLJ for metal and carbon;
+ * carbon-carbon entries are not taped and read 1).  Recorded from the positions, or replayed; NULL
+ * with the tape off.  *off receives the first entry of each row. */
+static unsigned char *FN(cut_tape)(const orc_params *p, const REAL *pos4, const int *NN, const int *NBL, int nRows,
+                                   size_t **off) {
+    if (!tape_mode()) return NULL;
+    const int nMe = p->nMolMe;
+    const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
+    size_t *o = (size_t *)malloc(sizeof(size_t) * ((size_t)nRows + 1));
+    o[0] = 0;
+    for (int i = 0; i < nRows; ++i) o[i + 1] = o[i] + (size_t)NN[i];
+    const size_t n = o[nRows], nw = (n + 31) / 32;
+    unsigned char *cut = (unsigned char *)malloc(n ? n : 1);
+    int *w = (int *)calloc(nw ? nw : 1, sizeof(int));
+    if (tape_mode() == 2) {
+        tape_pull(w, nw);
+        for (size_t q = 0; q < n; ++q) cut[q] = (unsigned char)(((unsigned)w[q / 32] >> (q % 32)) & 1u);
+    } else {
+        for (int i = 0; i < nRows; ++i)
+            for (int j = 0; j < NN[i]; ++j) {
+                const int k = NBL[(size_t)j * p->nMol + i];
+                REAL dx = wrap1(pos4[4 * i] - pos4[4 * k], Lx);
+                REAL dy = wrap1(pos4[4 * i + 1] - pos4[4 * k + 1], Ly);
+                REAL dz = wrap1(pos4[4 * i + 2] - pos4[4 * k + 2], Lz);
+                REAL rr = dx * dx + dy * dy + dz * dz;
+                const int mi = i < nMe, mk = k < nMe;
+                int in = 1;
+                if (mi && mk) in = rr < (REAL)p->rrCutEam;
+                else if (mi || mk) in = rr < (REAL)p->rrCutLJ;
+                const size_t q = o[i] + (size_t)j;
+                cut[q] = (unsigned char)in;
+                if (in) w[q / 32] = (int)((unsigned)w[q / 32] | (1u << (q % 32)));
+            }
+        tape_push(w, nw);
+    }
+    free(w);
+    *off = o;
+    return cut;
+}
+
 /* EamComputeRhoK (.cu:716-773): rho_i = sum over metal neighbours with r^2 < rrCutEam. */
 void FN(orc_rho)(const orc_params *p, const REAL *pos4, const int *NN, const int *NBL, REAL *rho) {
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
+    size_t *off = NULL;
+    unsigned char *cut = FN(cut_tape)(p, pos4, NN, NBL, p->nMolMe, &off);
     OMP_FOR
     for (int i = 0; i < p->nMolMe; ++i) {
         REAL sum = 0;
@@ -259,10 +364,13 @@ void FN(orc_rho)(const orc_params *p, const REAL *pos4, const int *NN, const int
             REAL dy = wrap1(pos4[4 * i + 1] - pos4[4 * k + 1], Ly);
             REAL dz = wrap1(pos4[4 * i + 2] - pos4[4 * k + 2], Lz);
             REAL rr = dx * dx + dy * dy + dz * dz;
-            if (rr < (REAL)p->rrCutEam && k < p->nMolMe) sum += eam_f(p, SQRT(rr));
+            const int in = cut ? cut[off[i] + (size_t)j] : rr < (REAL)p->rrCutEam;
+            if (in && k < p->nMolMe) sum += eam_f(p, SQRT(rr));
         }
         rho[i] = sum;
     }
+    free(cut);
+    free(off);
 }
 
 /* Lennard-Jones pair (.cu:880-884): returns the force factor, energy in *u. */
@@ -284,6 +392,18 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
     const REAL z0 = p->z_0, rrC = p->rrCutC, mu_r = p->mu_r, mu_p = p->mu_p, mu_t = p->mu_theta,
                th0 = p->theta_0;
+    int *br = NULL; /* F(rho) branches from the decision tape (record or replay), else from rho */
+    if (tape_mode() && nMe) {
+        br = (int *)malloc(sizeof(int) * (size_t)nMe);
+        if (tape_mode() == 2) {
+            tape_pull(br, (size_t)nMe);
+        } else {
+            for (int i = 0; i < nMe; ++i) br[i] = eam_branch(p, rho[i]);
+            tape_push(br, (size_t)nMe);
+        }
+    }
+    size_t *off = NULL;
+    unsigned char *cut = FN(cut_tape)(p, pos4, NN, NBL, nMol, &off);
     OMP_FOR
     for (int i = 0; i < nMol; ++i) {
         const int metal = i < nMe;
@@ -291,7 +411,7 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
         const REAL Ax = pos4[4 * i], Ay = pos4[4 * i + 1], Az = pos4[4 * i + 2];
         const int nn = NN[i];
         const REAL rhoi = metal ? rho[i] : 0;
-        const REAL dFi = metal ? eam_dF(p, rhoi) : 0;
+        const REAL dFi = metal ? eam_dF_br(p, rhoi, br ? br[i] : -1) : 0;
         REAL rnn[3][3], absRnn[3], deltaZ[3];
         int counti = 0;
         for (int j = 0; j < nn; ++j) {
@@ -301,21 +421,23 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
             REAL rr = dx * dx + dy * dy + dz * dz;
             if (metal) {
                 if (k < nMe) { /* EAM (.cu:857-877) */
+                    const int in = cut ? cut[off[i] + (size_t)j] : rr < (REAL)p->rrCutEam;
                     REAL r = SQRT(rr);
                     REAL fc = -eam_dphi(p, r);
                     REAL u = eam_phi(p, r);
-                    fc = fc - eam_df(p, r) * (dFi + eam_dF(p, rho[k]));
+                    fc = fc - eam_df(p, r) * (dFi + eam_dF_br(p, rho[k], br ? br[k] : -1));
                     fc = fc / r;
-                    if (rr >= (REAL)p->rrCutEam) fc = 0;
+                    if (!in) fc = 0;
                     fx += fc * dx;
                     fy += fc * dy;
                     fz += fc * dz;
                     wSum += fc * rr;
-                    if (rr < (REAL)p->rrCutEam) uSum += u;
+                    if (in) uSum += u;
                 } else { /* LJ with carbon (.cu:878-896) */
+                    const int in = cut ? cut[off[i] + (size_t)j] : rr < (REAL)p->rrCutLJ;
                     REAL u, fc = lj_pair(p, rr, &u);
-                    if (rr >= (REAL)p->rrCutLJ) fc = 0;
-                    if (rr < (REAL)p->rrCutLJ) uSum += u;
+                    if (!in) fc = 0;
+                    if (in) uSum += u;
                     fx += fc * dx;
                     fy += fc * dy;
                     fz += fc * dz;
@@ -325,9 +447,10 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
                 continue;
             }
             if (k < nMe) { /* carbon: LJ with metal (.cu:900-917) */
+                const int in = cut ? cut[off[i] + (size_t)j] : rr < (REAL)p->rrCutLJ;
                 REAL u, fc = lj_pair(p, rr, &u);
-                if (rr >= (REAL)p->rrCutLJ) fc = 0;
-                if (rr < (REAL)p->rrCutLJ) uSum += u;
+                if (!in) fc = 0;
+                if (in) uSum += u;
                 fx += fc * dx;
                 fy += fc * dy;
                 fz += fc * dz;
@@ -412,7 +535,7 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
             acc[3 * i] = fx / (REAL)p->massMe;
             acc[3 * i + 1] = fy / (REAL)p->massMe;
             acc[3 * i + 2] = fz / (REAL)p->massMe;
-            pos4[4 * i + 3] = K(0.5) * uSum + eam_F(p, rhoi);
+            pos4[4 * i + 3] = K(0.5) * uSum + eam_F_br(p, rhoi, br ? br[i] : -1);
             if (fric) fric[i] = frx;
         } else {
             acc[3 * i] = fx;
@@ -422,6 +545,9 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
         }
         if (vir) vir[i] = K(0.5) * wSum;
     }
+    free(br);
+    free(cut);
+    free(off);
 }
 
 void FN(orc_forces)(const orc_params *p, REAL *pos4, const int *NN, const int *NBL, const REAL *rho,
@@ -728,6 +854,8 @@ static void FN(orc_insert)(const orc_params *p, REAL *pos4, REAL *vel, int dep, 
  * each metal-carbon pair within rrCutLJ once (an observable of the adjoint; no reference output). */
 REAL FN(orc_adhesion)(const orc_params *p, const REAL *pos4, const int *NN, const int *NBL) {
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
+    size_t *off = NULL;
+    unsigned char *cut = FN(cut_tape)(p, pos4, NN, NBL, p->nMolMe, &off);
     REAL sum = 0;
     for (int i = 0; i < p->nMolMe; ++i) {
         for (int j = 0; j < NN[i]; ++j) {
@@ -739,10 +867,32 @@ REAL FN(orc_adhesion)(const orc_params *p, const REAL *pos4, const int *NN, cons
             REAL rr = dx * dx + dy * dy + dz * dz;
             REAL u;
             (void)lj_pair(p, rr, &u);
-            if (rr < (REAL)p->rrCutLJ) sum += u;
+            if (cut ? cut[off[i] + (size_t)j] : rr < (REAL)p->rrCutLJ) sum += u;
         }
     }
+    free(cut);
+    free(off);
     return sum;
+}
+
+/* A neighbour-list build (bin + list) or, replaying the decision tape, the recorded list. */
+static int FN(list_build)(const orc_params *p, REAL *pos4, int *cellOf, int *count, int *slots, int *NN, int *NBL) {
+    const int nMol = p->nMol;
+    if (tape_mode() == 2) {
+        tape_pull(NN, (size_t)nMol);
+        for (size_t q = 0; q < (size_t)nMol * p->iNebrMax; ++q) NBL[q] = -1;
+        for (int i = 0; i < nMol; ++i)
+            for (int j = 0; j < NN[i]; ++j) tape_pull(&NBL[(size_t)j * nMol + i], 1);
+        return 0;
+    }
+    int err = FN(orc_bin)(p, pos4, cellOf, count, slots);
+    err |= FN(orc_nebr)(p, pos4, count, slots, NN, NBL);
+    if (tape_mode() == 1) {
+        tape_push(NN, (size_t)nMol);
+        for (int i = 0; i < nMol; ++i)
+            for (int j = 0; j < NN[i]; ++j) tape_push(&NBL[(size_t)j * nMol + i], 1);
+    }
+    return err;
 }
 
 /* The step loop of DoComputationsW (.cu:2141-2489) with device work restated serially; adh[s]
@@ -776,11 +926,15 @@ static int FN(run_impl)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, i
             }
         }
         FN(orc_leapfrog)(p, 1, pos4, vel, acc);
-        if (*nebrNow || !haveList) {
+        int rebuild = *nebrNow || !haveList;
+        if (tape_mode() == 2)
+            tape_pull(&rebuild, 1);
+        else if (tape_mode() == 1)
+            tape_push(&rebuild, 1);
+        if (rebuild) {
             *nebrNow = 0;
             dh = 0;
-            err |= FN(orc_bin)(p, pos4, cellOf, count, slots);
-            err |= FN(orc_nebr)(p, pos4, count, slots, NN, NBL);
+            err |= FN(list_build)(p, pos4, cellOf, count, slots, NN, NBL);
             haveList = 1;
             rec.rebuilt = 1;
         }
@@ -858,8 +1012,7 @@ int FN(orc_window)(const orc_params *p, REAL *pos4, REAL *vel, REAL *acc, int ns
     REAL *rho = (REAL *)calloc((size_t)(nMe ? nMe : 1), sizeof(REAL));
     REAL *fric = (REAL *)calloc((size_t)(nMe ? nMe : 1), sizeof(REAL));
     REAL *adh = (REAL *)calloc((size_t)(nsteps > 0 ? nsteps : 1), sizeof(REAL));
-    int err = FN(orc_bin)(p, pos4, cellOf, count, slots);
-    err |= FN(orc_nebr)(p, pos4, count, slots, NN, NBL);
+    int err = FN(list_build)(p, pos4, cellOf, count, slots, NN, NBL);
     if (nMe) FN(orc_rho)(p, pos4, NN, NBL, rho);
     FN(orc_forces)(p, pos4, NN, NBL, rho, acc, fric);
     int nebrNow = 1;

@@ -7,7 +7,8 @@
       batched engine bitwise equal to the replica run alone; forces of the smallest and the largest
       replica against the FP64 oracle.
   differentiation (8,064 C + 561 Ni icosahedron, 600 K, equilibrated 2,000 steps): gradients of a
-      200-step window checkpointed every 20 steps against the FP64 tangent (1e-4 relative).
+      200-step window checkpointed every 20 steps against the FP64 tangent and against central
+      differences at h = 1e-4 with the discrete decisions frozen (1e-4 relative, all 8 components).
 """
 import numpy as np
 import pytest
@@ -141,3 +142,27 @@ def test_differentiation_window_against_fp64_tangent(gpu):
     rel = np.abs(g - dJ) / np.abs(dJ)
     print(f"[configs] icosahedron: gradient rel. error {rel.max():.2e}", flush=True)
     assert rel.max() <= 1e-4, (g, dJ)
+    # Central differences at h = 1e-4 of the FP64 window with the unperturbed run's decisions held
+    # (rebuild steps, lists, cutoff bits, F(rho) branches): the map the adjoint differentiates.  Free
+    # differences at this step miss by up to ~0.6 (pairs cross the LJ cutoff inside the window).
+    print("[configs] icosahedron: frozen central differences, h = 1e-4", flush=True)
+    h = 1e-4
+    o.freeze("record")
+    J0 = o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)[0]
+    assert np.array_equal(J0, J)
+    fd = np.zeros((2, 4))
+    for k in range(4):
+        Js = []
+        for sgn in (1, -1):
+            sc = np.ones(4)
+            sc[k] = 1 + sgn * h
+            o.set_scale(sc)
+            o.freeze("replay")
+            Js.append(o.window(r0, v0, window, step0=equil, rebuild_every=rebuild)[0])
+            assert o.freeze_ok(), k
+        fd[:, k] = (Js[0] - Js[1]) / (2 * h)
+    o.set_scale(np.ones(4))
+    o.freeze("off")
+    rel_fd = np.abs(g - fd) / np.abs(fd)
+    print(f"[configs] icosahedron: GPU gradient vs frozen differences, rel. error {rel_fd.max():.2e}", flush=True)
+    assert rel_fd.max() <= 1e-4, (g, fd)

@@ -128,3 +128,62 @@ def test_tangent_force_jacobian_matches_central_differences(tiny):
     # the adhesion energy is linear in epsilon and carries no EAM parameter
     assert dadhp[0] == pytest.approx(adh, rel=1e-12)
     assert dadhp[2] == 0 and dadhp[3] == 0
+
+
+def frozen_window_fd(o, r0, v0, nsteps, h, **kw):
+    """Central differences of the window with every discrete decision of the unperturbed run held
+    (orc_freeze_f64): rebuild steps, lists, cutoff bits, F(rho) branches."""
+    o.freeze("record")
+    J0 = o.window(r0, v0, nsteps, **kw)[0]
+    fd = np.zeros((2, 4))
+    for k in range(4):
+        Js = []
+        for sgn in (1, -1):
+            sc = np.ones(4)
+            sc[k] = 1 + sgn * h
+            o.set_scale(sc)
+            o.freeze("replay")
+            Js.append(o.window(r0, v0, nsteps, **kw)[0])
+            assert o.freeze_ok(), k
+        fd[:, k] = (Js[0] - Js[1]) / (2 * h)
+    o.set_scale(np.ones(4))
+    o.freeze("off")
+    return J0, fd
+
+
+def test_decision_tape_replays_the_recorded_run(tiny):
+    """Replaying the tape at the recorded parameters reproduces the run bit for bit; a replay that
+    runs longer than the recording reports it."""
+    s, o = tiny
+    r0, v0 = s.r.astype(np.float64), s.v.astype(np.float64)
+    n = o.freeze("record")
+    J, r, v, a, err = o.window(r0, v0, 60)
+    assert o.freeze("replay") > n == 0
+    Jr, rr, vr, ar, errr = o.window(r0, v0, 60)
+    assert o.freeze_ok()
+    assert np.array_equal(J, Jr) and np.array_equal(r, rr) and np.array_equal(v, vr) and np.array_equal(a, ar)
+    o.freeze("replay")
+    o.window(r0, v0, 61)
+    assert not o.freeze_ok()
+    o.freeze("replay")
+    o.window(r0, v0, 30)
+    assert not o.freeze_ok()  # a shorter replay leaves part of the tape unread
+    o.freeze("off")
+    assert np.array_equal(o.window(r0, v0, 60)[0], J)
+
+
+def test_frozen_differences_at_a_large_step_match_the_tangent(tiny):
+    """Over 200 steps at h = 1e-3 pairs cross the LJ cutoff and lists change between the perturbed
+    runs, so free central differences miss the tangent by ~5e-2; with the decisions frozen they match
+    it to O(h^2)."""
+    s, o = tiny
+    r0, v0 = s.r.astype(np.float64), s.v.astype(np.float64)
+    n = 200
+    _, dJ, *_ = o.tangent_window(r0, v0, n, pdir=np.eye(4))
+    J0, fd = frozen_window_fd(o, r0, v0, n, 1e-3)
+    assert np.array_equal(J0, o.window(r0, v0, n)[0])
+    scale = np.abs(dJ).max(axis=1, keepdims=True)
+    # O(h^2): the sigma direction enters as sigma^12, whose third derivative gives ~h^2 12^2/6 ~ 2e-5
+    assert (np.abs(fd - dJ) / scale).max() < 1e-4, (fd, dJ)
+    free = window_fd(o, r0, v0, n, 1e-3)
+    assert (np.abs(free - dJ) / scale).max() > 1e-2
