@@ -119,9 +119,19 @@ def nve_drift(config, steps, oracle_steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    o = oracle.Oracle(p, "f64")
-    err, _, _, _, recs = o.run(s.r, s.v, s.a, oracle_steps, 0, 1, 0.0, 10)
-    out["fp64_oracle_first_" + str(oracle_steps)] = slope([x["totEnergy"] for x in recs])
+    # the FP64 oracle on OpenMP threads (bitwise the serial one), 500 steps per call
+    o = oracle.Oracle(p, "f64", library=oracle.threaded_f64_lib())
+    r, v, a = s.r.astype(np.float64), s.v.astype(np.float64), s.a.astype(np.float64)
+    tot, nebr, disp = [], 1, 0.0
+    t0 = time.perf_counter()
+    while len(tot) < oracle_steps:
+        n = min(500, oracle_steps - len(tot))
+        err, r, v, a, recs = o.run(r, v, a, n, len(tot), nebr, disp, 10)
+        assert err == 0
+        nebr, disp = recs[-1]["nebrNow"], recs[-1]["dispHi"]
+        tot += [x["totEnergy"] for x in recs]
+        log(f"[bench] NVE FP64 oracle: {len(tot)} steps, {time.perf_counter() - t0:.0f} s")
+    out["fp64_oracle_first_" + str(oracle_steps)] = slope(tot)
     log(f"[bench] NVE FP64 oracle ({oracle_steps} steps): {out['fp64_oracle_first_' + str(oracle_steps)]:.4e}")
     return out
 

@@ -50,9 +50,11 @@ class OrcRecord(C.Structure):
 
 
 OMP_LIB = os.path.join(HERE, "_build", "liboracle_omp.so")  # CPU-baseline timing build (FP32, OpenMP)
+OMP64_LIB = os.path.join(HERE, "_build", "liboracle_omp64.so")  # FP64 oracle on OpenMP threads, bitwise lib()'s
 
 _lib = None
 _omp = None
+_omp64 = None
 
 
 def build():
@@ -69,6 +71,18 @@ def timing_lib():
         _omp = C.CDLL(OMP_LIB)
         _declare(_omp, ("f32",))
     return _omp
+
+
+def threaded_f64_lib():
+    """The FP64 oracle with OpenMP per-atom loops, built with the checker's flags: its *_f64 results
+    are bitwise those of lib() (each atom's sums keep their order); for long FP64 runs."""
+    global _omp64
+    if _omp64 is None:
+        if not os.path.exists(OMP64_LIB):
+            build()
+        _omp64 = C.CDLL(OMP64_LIB)
+        _declare(_omp64, ("f64",))
+    return _omp64
 
 
 def lib():

@@ -1,5 +1,7 @@
 """The benchmark workloads at their full sizes on the GPU, against the oracle.
 
+  medium (40,320 C + 4,033 Ni, 900 K): forces and energy after 10,000 device steps, each form against
+      the oracle of its precision.
   large (1,003,520 C + 100,000 Ni, 1,000 K): mixed-form forces and energy against the FP64 oracle
       (e_F <= 1e-5, energy <= 1e-7 relative) at the initial state and after 300 device steps; the
       plain form against the FP32 restatement at the initial state (e_F <= 1e-3: see the test).
@@ -81,6 +83,34 @@ def test_large_forces_after_300_steps(gpu, large):
     print(f"[configs] large after 300 steps: e_F {e_f(acc, a64):.3e}", flush=True)
     assert e_f(acc, a64) <= 1e-5
     assert abs(e.sum() - e64.sum()) <= 1e-7 * abs(e64.sum())
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_medium_forces_at_step_10000(gpu, prec):
+    """medium (40,320 C + 4,033 Ni, 900 K): forces and energy of the state after 10,000 device steps
+    against the oracle at that state - the mixed form against FP64 (e_F <= 1e-5, energy <= 1e-7),
+    the plain form against the FP32 restatement (e_F <= 2e-4, energy <= 1e-6)."""
+    s = sg.config_system("medium", "config")
+    p = s.params
+    print(f"[configs] medium: 10,000 {prec} steps", flush=True)
+    with sg.Engine(p, precision=prec, rebuild_every=rebuild_every("config"), records_capacity=1000) as eng:
+        eng.upload(s.r, s.v, s.a)
+        eng.set_run_state(0, 1, 0.0)
+        for _ in range(10):
+            recs = eng.run(1000)
+        r, _, _ = eng.download()
+    assert recs[-1]["step"] == 10000 and recs[-1]["errors"] == 0
+    o = oracle.Oracle(p, "f64" if prec == "mixed" else "f32")
+    rin = r if prec == "mixed" else r.astype(np.float32)
+    err, nn, nbl = o.neighbour_list(rin.astype(np.float32) if prec == "mixed" else rin)
+    assert err == 0
+    aref, eref, _ = o.forces(rin, nn, nbl)
+    aref, eref = aref.astype(np.float64), eref.astype(np.float64)
+    acc, e, _ = engine_forces(p, rin, prec)
+    ef = e_f(acc, aref)
+    print(f"[configs] medium {prec} at step 10,000: e_F {ef:.3e}", flush=True)
+    assert ef <= (1e-5 if prec == "mixed" else 2e-4)
+    assert abs(e.sum() - eref.sum()) <= (1e-7 if prec == "mixed" else 1e-6) * abs(eref.sum())
 
 
 def test_ensemble_of_64_replicas(gpu):

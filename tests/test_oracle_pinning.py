@@ -525,3 +525,16 @@ def test_deposition_schedule_and_parking():
             assert r[n, 2] == pytest.approx(park - float(Lz), rel=1e-6)
             assert np.all(v[n] == 0.0)
     assert dep == 15  # groups at steps 7, 11, 15 (the last group exceeds nMolMe, as in the reference)
+
+
+def test_threaded_fp64_oracle_is_bitwise_the_serial_one():
+    """liboracle_omp64 (per-atom loops on OpenMP threads, the checker's flags) gives the serial FP64
+    oracle's run bit for bit: the long FP64 reference runs may use it."""
+    s = sg.config_system("small", "config")
+    o = oracle.Oracle(s.params, "f64")
+    ot = oracle.Oracle(s.params, "f64", library=oracle.threaded_f64_lib())
+    a = o.run(s.r, s.v, s.a, 40, 0, 1, 0.0, 10)
+    b = ot.run(s.r, s.v, s.a, 40, 0, 1, 0.0, 10)
+    assert a[0] == b[0] == 0
+    assert all(np.array_equal(x, y) for x, y in zip(a[1:4], b[1:4]))
+    assert a[4] == b[4]
