@@ -122,6 +122,11 @@ int sgk_upload_f64(sgk_ctx *ctx, const double *r, const double *v, const double 
 int sgk_download_f64(sgk_ctx *ctx, double *r, double *v, double *a);
 /* Step counter, rebuild flag and displacement accumulator of the run state. */
 int sgk_set_run_state(sgk_ctx *ctx, int stepCount, int nebrNow, double dispHi);
+/* Shear state of the run (regime 2): the shear force per metal atom, and the metal centre-of-mass x
+ * and centre-of-mass velocity x of the last step, which the next step's shear reads (ApplyShearK
+ * and the increment rule, SurfaceGrowth.cu:517-534, 2398-2404).  For restarts; a new context starts
+ * at 0. */
+int sgk_set_shear_state(sgk_ctx *ctx, double shear, double comX, double cmVelX);
 
 /* --- stages ---------------------------------------------------------------------------------- */
 /* Cell binning (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609): cell index per atom with the

@@ -162,6 +162,7 @@ EXPORTED_SYMBOLS = [
     "AccumProps",
     "sgk_default_config", "sgk_create", "sgk_destroy", "sgk_last_error", "sgk_device_count",
     "sgk_upload", "sgk_download", "sgk_upload_f64", "sgk_download_f64", "sgk_set_run_state",
+    "sgk_set_shear_state",
     "sgk_bin", "sgk_get_cells", "sgk_build_nebr", "sgk_build_nl_order", "sgk_get_nebr", "sgk_get_carbon_lists",
     "sgk_compute_rho", "sgk_compute_forces", "sgk_get_rho", "sgk_get_forces", "sgk_leapfrog",
     "sgk_thermostat", "sgk_eval_props", "sgk_rdf_clear", "sgk_rdf_accumulate", "sgk_rdf_get",
@@ -218,6 +219,7 @@ def _proto(lib):
         "sgk_upload_f64": (_i, [vp, vp, vp, vp]),
         "sgk_download_f64": (_i, [vp, vp, vp, vp]),
         "sgk_set_run_state": (_i, [vp, _i, _i, C.c_double]),
+        "sgk_set_shear_state": (_i, [vp, C.c_double, C.c_double, C.c_double]),
         "sgk_bin": (_i, [vp]),
         "sgk_get_cells": (_i, [vp, vp, vp, vp]),
         "sgk_build_nebr": (_i, [vp]),

@@ -28,6 +28,59 @@ char *fail(char *szPdbPath, const char *what, const char *detail) {
 
 long long next_multiple(long long s, long long k) { return (s / k + 1) * k; }
 
+// Restart from backup (row 22, bStartBckup).  The reference parses the row (SurfaceGrowth.cpp:775,
+// stored at :378) and writes backups (SurfaceGrowth.cu:2457-2480, alternately to szBckup0/1) but
+// never reads one; here the run continues from the newer of the two files (the larger stepCount in
+// its SimParams).  Restored: positions, velocities, accelerations and the run state the backup's
+// SimParams carries (step, displacement, deposition counter, averaging accumulators, shear, elapsed
+// time).  The first step rebuilds the neighbour list, which a backup does not hold; where the
+// continuous run rebuilds at that step too (a list rebuilt every k steps, backups every multiple of
+// k), the restarted run is the continuous one, bit for bit.  The RDF histogram and the diffusion buffers are not in a backup: both
+// start empty.  The input's other fields (step limit, intervals, outputs) stay those of this run.
+// Returns 0, or the message.
+const char *restore_backup(SimParams &p, sg_float4 *hr, sg_float3 *hv, sg_float3 *ha) {
+    const char *paths[2] = {p.szBckup0, p.szBckup1};
+    int best = -1;
+    SimParams b;
+    for (int k = 0; k < 2; ++k) {
+        FILE *f = std::fopen(paths[k], "rb");
+        if (!f) continue;
+        SimParams h;
+        const bool ok = std::fread(&h, sizeof(SimParams), 1, f) == 1;
+        std::fclose(f);
+        if (ok && (best < 0 || h.stepCount > b.stepCount)) {
+            best = k;
+            b = h;
+        }
+    }
+    if (best < 0) return "Cannot open backup file!";
+    if (b.nMol != p.nMol || b.nMolMe != p.nMolMe || b.iRegime != p.iRegime || b.iMaterial != p.iMaterial ||
+        b.region.x != p.region.x || b.region.y != p.region.y || b.region.z != p.region.z ||
+        b.cells.x != p.cells.x || b.cells.y != p.cells.y || b.cells.z != p.cells.z)
+        return "The backup file does not match the input!";
+    if (!sgReadBackup(paths[best], &b, hr, hv, ha, p.nMol)) return "Cannot read backup file!";
+    p.stepCount = b.stepCount;
+    p.nebrNow = 1;  // the list is not in a backup: the first step builds it
+    p.dispHi = b.dispHi;
+    p.nMolDeposited = b.nMolDeposited;
+    p.vSum = b.vSum;
+    p.kinEnergy = b.kinEnergy;
+    p.totEnergy = b.totEnergy;
+    p.potEnergy = b.potEnergy;
+    p.oneStep = b.oneStep;
+    p.centerOfMass = b.centerOfMass;
+    p.cmVel = b.cmVel;
+    p.frictForce = b.frictForce;
+    p.particleSize = b.particleSize;
+    p.shear = b.shear;
+    p.totalShear = b.totalShear;
+    p.totalTime = b.totalTime;
+    p.countRdf = 0;
+    p.countDiffuseAv = 0;
+    p.moreCycles = p.stepCount < p.stepLimit ? 1 : 0;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -94,6 +147,10 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
                       char *szPdbPath) {
     SimParams &p = *hparams;
     const SgOptions &o = sg::options();
+    if (p.bStartBckup) {
+        if (!hr || !hv || !ha) return fail(szPdbPath, "Cannot restart from backup: ", "null state arrays");
+        if (const char *e = restore_backup(p, hr, hv, ha)) return fail(szPdbPath, e, nullptr);
+    }
     std::unique_ptr<sg::Diffusion> diff;
     FILE *fileDiffuse = nullptr;
     if (p.iRegime == 2) {
@@ -123,6 +180,10 @@ char *DoComputationsW(sg_float4 *hr, sg_float3 *hv, sg_float3 *ha, SimParams *hp
     if (sgk_upload(ctx, hr, hv, ha) != SGK_OK) return bail("Problems with memory allocation! Exception: ");
     if (sgk_set_run_state(ctx, p.stepCount, p.nebrNow, p.dispHi) != SGK_OK ||
         sgk_set_deposited(ctx, p.nMolDeposited) != SGK_OK)
+        return bail("Problems with memory allocation! Exception: ");
+    // a restarted shear run continues with its shear force; the last step's centre of mass and its
+    // velocity are the backup's FP32 values
+    if (p.bStartBckup && sgk_set_shear_state(ctx, p.shear, p.centerOfMass.val, p.cmVel.val) != SGK_OK)
         return bail("Problems with memory allocation! Exception: ");
     if (p.bPdb != 0) sg::create_pdb_file(szPdbPath, p, hr);
 

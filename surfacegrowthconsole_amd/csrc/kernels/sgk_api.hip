@@ -608,6 +608,17 @@ int sgk_set_run_state(sgk_ctx *c, int stepCount, int nebrNow, double dispHi) {
     return SGK_OK;
 }
 
+int sgk_set_shear_state(sgk_ctx *c, double shear, double comX, double cmVelX) {
+    if (!c) return SGK_E_ARG;
+    SGK_HIP(c, hipStreamSynchronize(c->stream));
+    SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
+    c->hostState->shear = shear;
+    c->hostState->comX = comX;
+    c->hostState->cmVelX = cmVelX;
+    SGK_HIP(c, hipMemcpy(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice));
+    return SGK_OK;
+}
+
 // ---- stages ------------------------------------------------------------------------------------------
 int sgk_bin(sgk_ctx *c) {
     if (!c) return SGK_E_ARG;
@@ -971,6 +982,14 @@ int sgk::run_steps(sgk_ctx *c, int nsteps, int rdfEvery, sgk_record *records, do
         return fail(c, SGK_E_ARG, "nsteps exceeds records_capacity; call sgk_run in chunks");
     if (adh && (!c->adj || c->adj->every < nsteps)) return fail(c, SGK_E_STATE, "no adhesion-energy buffer for the run");
     const std::string flags = sgk::step_flag_string(c, c->hostStep, nsteps, rdfEvery, adh != nullptr);
+    if (!c->haveList) {  // positions uploaded since the last list, or none built: build it at the first step
+        SGK_HIP(c, hipStreamSynchronize(c->stream));
+        SGK_HIP(c, hipMemcpy(c->hostState, c->state, sizeof(DevState), hipMemcpyDeviceToHost));
+        if (!c->hostState->nebrNow) {
+            c->hostState->nebrNow = 1;
+            SGK_HIP(c, hipMemcpy(c->state, c->hostState, sizeof(DevState), hipMemcpyHostToDevice));
+        }
+    }
     SGK_HIP(c, hipEventRecord(c->ev0, c->stream));
     // the list-build order of nl_scheme 3 follows the cells of the current positions; it decides
     // only which lane builds which atom's lists, so it is refreshed now and then, outside the graph

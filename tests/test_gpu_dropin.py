@@ -133,3 +133,44 @@ def test_console_driver_deposition(gpu, tmp_path):
     z = np.array([float(l[47:55]) for l in pdb[:40]])
     # at step 30 the groups of steps 25 (8 atoms) are in: the other 32 print at 2 Lz
     assert np.sum(np.abs(z - lz2) < 0.01) == 32
+
+
+def test_console_driver_restart_from_backup(gpu, tmp_path):
+    """Row 22 (start from backup): a run to step 200 with backups every 100 steps, then the same input
+    with step limit 300 and row 22 set, continues from the newer backup (step 200) and gives the
+    continuous 300-step run's sg_*.txt rows and final backup, bit for bit (plain FP32, list rebuilt
+    every 10 steps, so both runs rebuild at step 201)."""
+    base = [2, 3, 5, 6, 0, 300, 201, 0.023, 2.85, 300, 0.005, 50, 0, 0.0, 0, 0.03, 40, 1100, 100000, 50, 1, 0]
+    extra = ["seed=1", "geometry=sphere", "gap=2.5", "backup_every=100", "rebuild=10"]
+    name = "sg_sh_x5_y6_Me201_Eq0_C0_Av50_Pdb100000_T300Ni.txt"
+    cont = tmp_path / "continuous"
+    cont.mkdir()
+    run_driver(cont, base, extra)
+    part = tmp_path / "restart"
+    part.mkdir()
+    first = list(base)
+    first[9] = 200
+    run_driver(part, first, extra)
+    _, rows200 = rows_of(part / name)
+    assert rows200[:, 0].tolist() == [50, 100, 150, 200]
+    again = list(base)
+    again[21] = 1
+    run_driver(part, again, extra)
+    _, rows = rows_of(part / name)
+    _, ref = rows_of(cont / name)
+    assert rows[:, 0].tolist() == [250, 300]
+    keep = [c for c in range(ref.shape[1]) if c != 8]  # column 8 is the wall time per step
+    assert ref[:, 0].tolist() == [50, 100, 150, 200, 250, 300]
+    assert np.array_equal(rows[:, keep], ref[4:, keep])
+    assert np.array_equal(rows200[:, keep], ref[:4, keep])
+    # the step-300 backups (bckup0.sugr in both): SimParams header, then r, v, a byte for byte
+    n = 1161
+    hdr = os.path.getsize(cont / "bckup0.sugr") - n * (16 + 12 + 12)
+    a, b = open(cont / "bckup0.sugr", "rb").read(), open(part / "bckup0.sugr", "rb").read()
+    assert len(a) == len(b) and a[hdr:] == b[hdr:]
+    # a backup of another system is refused with the reference's error convention
+    other = list(again)
+    other[6] = 55
+    write_input(part / "sugr_in.txt", other, extra)
+    out = subprocess.run([DRIVER], cwd=part, capture_output=True, text=True, timeout=150)
+    assert out.returncode != 0 and "does not match" in out.stdout
