@@ -4,8 +4,9 @@ A step is one full MD step of the hot path (leapfrog + boundary + binning when d
 when due, EAM density, fused EAM/LJ/graphene force pass with leapfrog part 2 and the step
 reductions, thermostat on its steps) over the whole system, with all state resident in HBM, plus
 the samples the large workload takes on the device: the radial distribution function (400 bins to
-12 A, every metal pair within range) and the centre-of-mass MSD buffers, each every --sample-every
-steps (default 10).  The step graph of the timed pattern is captured before the timed region.
+12 A, every metal pair within range) every stepRdf steps (50, the reference input's row 20) and the
+centre-of-mass MSD buffers every stepDiffuse steps (100); --rdf-every / --msd-every override them.
+The step graph of the timed pattern is captured before the timed region.
 
 Default workload: the large system - 1,003,520-atom periodic graphene sheet + 100,000-atom Ni fcc
 sphere 2.5 A above it, 1,000 K Maxwell-Boltzmann velocities, dt = 1 fs, Berendsen tau = 0.1 ps,
@@ -220,17 +221,18 @@ def b_alg(precision, f_me):
     return (96.0 if precision == "mixed" else 48.0) + 20.0 * f_me
 
 
-def time_form(sys_, precision, steps, warmup, rebuild, chunk, sample_every):
-    """Warm-up, then `steps` timed device-resident steps (RDF and MSD samples every `sample_every`
-    steps, complete-range RDF), then the same number again with HIP events around every launch for
-    the per-kernel durations.  The graph of the timed chunk is captured after the warm-up."""
+def time_form(sys_, precision, steps, warmup, rebuild, chunk, sample_every, msd):
+    """Warm-up, then `steps` timed device-resident steps (complete-range RDF samples every
+    `sample_every` steps, MSD samples every stepDiffuse steps if `msd`), then the same number again
+    with HIP events around every launch for the per-kernel durations.  The graph of the timed chunk
+    is captured after the warm-up."""
     import surfacegrowthconsole_amd as sg
 
     p = sys_.params
     eng = sg.Engine(p, precision=precision, rebuild_every=rebuild, records_capacity=chunk, rdf_complete=1)
     eng.upload(sys_.r, sys_.v, sys_.a)
     eng.set_run_state(0, 1, 0.0)
-    if sample_every:
+    if msd:
         eng.diffusion_enable(4)
     eng.rdf_clear()
     done = 0
@@ -426,7 +428,9 @@ def bench_adjoint(equil, window, every, fd_steps):
         "fd": {},
     }
     # Central differences free (each perturbed run takes its own decisions) and frozen (the decision
-    # tape of the unperturbed run replayed: rebuild steps, lists, cutoff bits, F(rho) branches).
+    # tape of the unperturbed run replayed: rebuild steps, lists, cutoff bits, F(rho) branches), on
+    # the threaded build of the FP64 oracle (bitwise the serial one).
+    o = oracle.Oracle(p, "f64", library=oracle.threaded_f64_lib())
     out["fd_frozen"] = {}
     if fd_steps:
         o.freeze("record")
@@ -462,8 +466,10 @@ def main():
     ap.add_argument("--config", default="large", choices=["small", "medium", "large", "icosa"])
     ap.add_argument("--variant", default="config", choices=["config", "reference"])
     ap.add_argument("--chunk", type=int, default=500)
-    ap.add_argument("--sample-every", type=int, default=10,
-                    help="RDF and centre-of-mass MSD sample interval in the timed steps (0: none)")
+    ap.add_argument("--rdf-every", type=int, default=-1,
+                    help="RDF sample interval in the timed steps (-1: the workload's stepRdf; 0: none)")
+    ap.add_argument("--msd-every", type=int, default=-1,
+                    help="centre-of-mass MSD sample interval (-1: the workload's stepDiffuse; 0: none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -509,16 +515,18 @@ def main():
     from surfacegrowthconsole_amd.configs import rebuild_every
 
     log(f"[bench] building {args.config}/{args.variant}")
-    extra = {"stepDiffuse": args.sample_every} if args.sample_every else {}
+    extra = {"stepDiffuse": args.msd_every} if args.msd_every > 0 else {}
     s = sg.config_system(args.config, args.variant, **extra)
     p = s.params
+    rdf_every = p.stepRdf if args.rdf_every < 0 else args.rdf_every
+    msd_every = 0 if args.msd_every == 0 else p.stepDiffuse
     rebuild = rebuild_every(args.variant)
     f_me = p.nMolMe / p.nMol
     res = {}
     for prec in ("mixed", "fp32"):
         log(f"[bench] {prec}: {args.warmup} warm-up + {args.steps} timed steps")
         wall, dev_ms, stats, r_end = time_form(s, prec, args.steps, args.warmup, rebuild, args.chunk,
-                                               args.sample_every)
+                                               rdf_every, msd_every > 0)
         res[prec] = dict(wall=wall, dev_ms=dev_ms, stats=stats, r_end=r_end)
         log(f"[bench] {prec}: {wall / args.steps * 1e3:.3f} ms/step, {p.nMol * args.steps / wall:.4e} atom-steps/s")
     m = res["mixed"]
@@ -538,10 +546,10 @@ def main():
         bound = "valu"
     kernels = {k: {"launches": v[0], "us_per_launch": round(v[1] / v[0] * 1e3, 3)} for k, v in m["stats"].items()}
     samples = {}
-    for k in ("rdf", "diffuse"):
+    for k, every in (("rdf", rdf_every), ("diffuse", msd_every)):
         if k in m["stats"]:
             samples[k] = {"us_per_sample": round(m["stats"][k][1] / m["stats"][k][0] * 1e3, 3),
-                          "every_steps": args.sample_every}
+                          "every_steps": every}
     lu = p.lengthU  # angstrom per reduced length unit
     out = {
         "metric": "atom-steps/s on 1 MI355X, mixed vs FP32 at <=1e-5 rel force err vs FP64",
@@ -565,9 +573,9 @@ def main():
                    "cutoffs_note": "reference rules: EAM cut at 1.45 a; LJ limited by the neighbour-list radius "
                                    "(rCutEam + skin), not 2.5 sigma",
                    "rdf": f"{p.sizeHistRdf} bins to {p.rangeRdf * lu:.1f} A, every metal pair in range, "
-                          f"every {args.sample_every} steps" if args.sample_every else "none",
-                   "msd": f"centre-of-mass buffers on the device (FP64), every {args.sample_every} steps"
-                          if args.sample_every else "none",
+                          f"every {rdf_every} steps" if rdf_every else "none",
+                   "msd": f"centre-of-mass buffers on the device (FP64), every {msd_every} steps"
+                          if msd_every else "none",
                    "parallelism": "single GPU"},
         "fp32": {"value": p.nMol * args.steps / res["fp32"]["wall"],
                  "ms_per_step": res["fp32"]["wall"] / args.steps * 1e3},

@@ -70,8 +70,10 @@ typedef struct sgk_config {
                              lists */
     int rdf_complete;     /* RDF: 0 the reference's 27-cell stencil of the last rebuilt cells (complete
                              only to one cell width); 1 every metal-metal pair within rangeRdf at the
-                             current positions (minimum image; rangeRdf must stay below half the box).
-                             No counterpart in the reference. */
+                             current positions (minimum image; rangeRdf must stay below half the box),
+                             over a grid of cells rangeRdf/2 wide; 2-6 the same over cells rangeRdf/k
+                             wide, cells beyond range culled (same histogram; on the 100,000-atom
+                             cluster 2 is the fastest).  No counterpart in the reference. */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with

@@ -82,6 +82,9 @@ def threaded_f64_lib():
             build()
         _omp64 = C.CDLL(OMP64_LIB)
         _declare(_omp64, ("f64",))
+        _omp64.orc_freeze_f64.restype = C.c_longlong
+        _omp64.orc_freeze_f64.argtypes = [C.c_int]
+        _omp64.orc_freeze_ok_f64.restype = C.c_int
     return _omp64
 
 

@@ -361,7 +361,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         SGK_TRY(dalloc(c, &c->ordCnt, nb, false));
         SGK_TRY(dalloc(c, &c->ordKey, nMol, false));
         SGK_TRY(dalloc(c, &c->ordSlot, nMol, false));
-        SGK_TRY(dalloc(c, &c->ordBlock, (nb + 4095) / 4096, false));
+        SGK_TRY(dalloc(c, &c->ordBlock, (nb + sgk::kOrdChunk - 1) / sgk::kOrdChunk, false));
     }
     if (p.keepFull) {
         SGK_TRY(dalloc(c, &c->fullNN, nMol, true));
@@ -370,7 +370,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const size_t nParts = (size_t)c->part2Off;
     SGK_TRY(dalloc(c, (char **)&c->partials, (nParts + nParts / 64 + 2) * sgk::partial_bytes(c), true));
     if (cfg.rdf_complete && p.nMolMe > 0) {
-        c->rdfGrid = sgk::rdf_grid(p);
+        c->rdfGrid = sgk::rdf_grid(p, cfg.rdf_complete == 1 ? 2 : cfg.rdf_complete);
         if (c->rdfGrid.ncell == 0) {
             g_create_error = "rdf_complete needs rangeRdf below half of every box edge";
             sgk_destroy(c);
@@ -378,7 +378,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
         }
         const size_t nb = (size_t)c->rdfGrid.ncell + 1;
         SGK_TRY(dalloc(c, &c->rdfCnt, nb, false));
-        SGK_TRY(dalloc(c, &c->rdfBlock, (nb + 4095) / 4096, false));
+        SGK_TRY(dalloc(c, &c->rdfBlock, (nb + sgk::kOrdChunk - 1) / sgk::kOrdChunk, false));
         SGK_TRY(dalloc(c, &c->rdfKey, nMe, false));
         SGK_TRY(dalloc(c, &c->rdfSlot, nMe, false));
         SGK_TRY(dalloc(c, &c->rdfPos, nMe, false));

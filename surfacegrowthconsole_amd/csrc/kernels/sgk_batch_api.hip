@@ -311,7 +311,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         BTRY(balloc(b, &b->ordCnt, (size_t)ordOff, false));
         BTRY(balloc(b, &b->ordKey, (size_t)atomOff, false));
         BTRY(balloc(b, &b->ordSlot, (size_t)atomOff, false));
-        BTRY(balloc(b, &b->ordBlock, (size_t)(ordOff + 4095) / 4096, false));
+        BTRY(balloc(b, &b->ordBlock, (size_t)(ordOff + sgk::kOrdChunk - 1) / sgk::kOrdChunk, false));
     }
     BTRY(balloc(b, (char **)&b->pos, (size_t)atomOff * (b->mixed ? 32 : 16), true));
     BTRY(balloc(b, (char **)&b->vel, (size_t)atomOff * 3 * Rb, true));

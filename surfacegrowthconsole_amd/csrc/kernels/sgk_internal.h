@@ -15,6 +15,10 @@
 
 namespace sgk {
 
+// Buckets per block of the two-pass counting-sort scan (k_ord_blocksum / k_ord_scan); the block-total
+// arrays hold one int per kOrdChunk buckets.
+constexpr int kOrdChunk = 1024;
+
 struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     unsigned long long uses = 0;
@@ -189,7 +193,7 @@ void build_nl_order(sgk_ctx *c);
 constexpr int kNlOrderSteps = 100;  // age (steps) at which sgk_run rebuilds the nl_scheme 3 list-build order
 void launch_commit_rebuild(sgk_ctx *c);
 void launch_rdf(sgk_ctx *c);
-RdfGrid rdf_grid(const DevParams &p);  // grid of the complete-range RDF (ncell 0: range too large)
+RdfGrid rdf_grid(const DevParams &p, int S);  // complete-range RDF grid, cells range/S (ncell 0: range too large)
 // sgk_force.hip
 void launch_rho(sgk_ctx *c);
 void launch_batch_rho(sgk_batch *b);

@@ -852,7 +852,7 @@ __global__ __launch_bounds__(256) void k_nlc(DevParams p, int nbM, const typenam
 // carbons in buckets [0, ncell], metal atoms in [ncell + 1, 2 ncell + 1], so that nlOrder[0, nC) are
 // the carbons and nlOrder[nC, nMol) the metal atoms.  It is built outside the step loop and ages as
 // atoms move; it decides which lanes build which atom's lists, never the lists.
-constexpr int kOrdChunk = 4096;  // buckets per scan block, 16 per thread
+// kOrdChunk (sgk_internal.h): buckets per scan block, 4 per thread, read as one 16-byte load
 
 // Bucket of atom n in the list-build order: its column-major cell, carbons first.
 __device__ __forceinline__ int ord_bucket(const DevParams &p, const float3 q, int n) {
@@ -895,13 +895,22 @@ __global__ __launch_bounds__(256) void k_ord_count(DevParams p, const typename P
 }
 
 // Total of each block of kOrdChunk buckets [0, nb).
+// The 4 counts of thread t of scan block b: [b kOrdChunk + 4t, +4), zero past nb.  Bucket arrays are
+// allocated 256-byte aligned, so the full groups are single 16-byte loads.
+__device__ __forceinline__ int4 ord_load4(const int *__restrict__ cnt, int nb, long long base) {
+    if (base + 3 < nb) return *(const int4 *)(cnt + base);
+    int4 v = make_int4(0, 0, 0, 0);
+    if (base < nb) v.x = cnt[base];
+    if (base + 1 < nb) v.y = cnt[base + 1];
+    if (base + 2 < nb) v.z = cnt[base + 2];
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_ord_blocksum(int nb, const int *__restrict__ cnt, int *__restrict__ blockSum) {
     __shared__ int sh[256 + 1];
-    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 16;
-    int sum = 0;
-    for (int j = 0; j < 16; ++j)
-        if (base + j < nb) sum += cnt[base + j];
-    sh[threadIdx.x] = sum;
+    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 4;
+    const int4 v = ord_load4(cnt, nb, base);
+    sh[threadIdx.x] = v.x + v.y + v.z + v.w;
     __syncthreads();
     block_scan4(sh, blockDim.x);
     if (threadIdx.x == 0) blockSum[blockIdx.x] = sh[blockDim.x];
@@ -917,21 +926,18 @@ __global__ __launch_bounds__(256) void k_ord_scan(int nb, int *__restrict__ cnt,
     block_scan4(sh, blockDim.x);
     pre = sh[blockDim.x];
     __syncthreads();
-    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 16;
-    int v[16], sum = 0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        v[j] = base + j < nb ? cnt[base + j] : 0;
-        sum += v[j];
-    }
-    sh[threadIdx.x] = sum;
+    const long long base = (long long)blockIdx.x * kOrdChunk + threadIdx.x * 4;
+    const int4 v = ord_load4(cnt, nb, base);
+    sh[threadIdx.x] = v.x + v.y + v.z + v.w;
     __syncthreads();
     block_scan4(sh, blockDim.x);
-    int run = pre + sh[threadIdx.x];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (base + j < nb) cnt[base + j] = run;
-        run += v[j];
+    const int r0 = pre + sh[threadIdx.x], r1 = r0 + v.x, r2 = r1 + v.y, r3 = r2 + v.z;
+    if (base + 3 < nb) {
+        *(int4 *)(cnt + base) = make_int4(r0, r1, r2, r3);
+    } else {
+        if (base < nb) cnt[base] = r0;
+        if (base + 1 < nb) cnt[base + 1] = r1;
+        if (base + 2 < nb) cnt[base + 2] = r2;
     }
 }
 
@@ -1086,11 +1092,13 @@ __global__ __launch_bounds__(256) void kb_rdf(BatchArgs a) {
 // ---- complete-range RDF (no counterpart in the reference) ------------------------------------
 // The reference stencil above finds every pair only up to one cell width.  This variant counts every
 // ordered metal-metal pair within rangeRdf at the current positions: metal atoms are sorted into a
-// grid of cells at least rangeRdf/2 wide (count, scan, scatter of float positions with the atom index
-// in w), then each atom's lanes walk the 5x5x5 cells around it.  A pair is counted once, from the atom
-// in the lower cell (or, in one cell, the lower index), and adds 2 to its bin - the reference's
-// histogram counts both orders.  Displacements use the single-image wrap, so rangeRdf must stay below
-// half the box; bins use the reference's formula.
+// grid of cells at least rangeRdf/S wide (count, scan, scatter of float positions with the atom index
+// in w), then each atom's lanes walk the (2S+1)^3 cells around it, skipping those whose box lies
+// beyond rangeRdf from the atom's cell (none for S = 2; S = 2 is the default and measured fastest on
+// a 100,000-atom cluster: finer cells cull more of the cube but cost more per cell).  A pair is counted once, from the atom in the lower cell
+// (or, in one cell, the lower index), and adds 2 to its bin - the reference's histogram counts both
+// orders.  Displacements use the single-image wrap, so rangeRdf must stay below half the box; bins
+// use the reference's formula.
 __device__ __forceinline__ int rdf_cell_axis(float x, float L, float iw, int n) {
     const int c = (int)((x + 0.5f * L) * iw);
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
@@ -1134,10 +1142,13 @@ __global__ __launch_bounds__(256) void k_rdfc_pairs(DevParams p, RdfGrid g, cons
     for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt / kRdfLanes, l = gt % kRdfLanes;
+    const int q = gt / kRdfLanes, l = gt % kRdfLanes;
     int edge = 0;
-    if (i < p.nMolMe) {
-        const float3 A = f3(pos[i]);
+    if (q < p.nMolMe) {
+        // atoms in cell order: the atoms of a wave share their stencil
+        const float4 me = sorted[q];
+        const int i = __float_as_int(me.w);
+        const float3 A = make_float3(me.x, me.y, me.z);
         const int ci = key[i];
         const int cz = ci / (g.nx * g.ny), cy = (ci / g.nx) % g.ny, cx = ci % g.nx;
         // stencil extents per axis: 2s+1 cells around the atom's, or the whole axis once
@@ -1149,6 +1160,12 @@ __global__ __launch_bounds__(256) void k_rdfc_pairs(DevParams p, RdfGrid g, cons
         const float range2 = p.rangeRdf2;
         for (int t = l; t < ncells; t += kRdfLanes) {
             const int ox = t % tx, oy = (t / tx) % ty, oz = t / (tx * ty);
+            // closest approach of the two cell boxes along each axis (none where the stencil spans
+            // the whole axis): beyond rangeRdf, the cell holds no pair in range
+            const float gx = tx < g.nx ? (float)max(abs(ox - g.sx) - 1, 0) * g.wx : 0.f;
+            const float gy = ty < g.ny ? (float)max(abs(oy - g.sy) - 1, 0) * g.wy : 0.f;
+            const float gz = tz < g.nz ? (float)max(abs(oz - g.sz) - 1, 0) * g.wz : 0.f;
+            if (gx * gx + gy * gy + gz * gz >= g.cull2) continue;
             int x = x0 + ox, y = y0 + oy, z = z0 + oz;
             x += x < 0 ? g.nx : (x >= g.nx ? -g.nx : 0);
             y += y < 0 ? g.ny : (y >= g.ny ? -g.ny : 0);
@@ -1186,16 +1203,17 @@ __global__ __launch_bounds__(256) void k_fill_i32(int *__restrict__ a, long long
 // ---- launchers ---------------------------------------------------------------------------------
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// The RDF grid of a system: cells at least rangeRdf/2 wide per axis (a pair within range then lies
-// within two cells); 0 cells when rangeRdf reaches half the box (the single-image wrap would miss
+// The RDF grid of a system: cells at least rangeRdf/S wide per axis (a pair within range then lies
+// within S cells); 0 cells when rangeRdf reaches half the box (the single-image wrap would miss
 // pairs).
-RdfGrid rdf_grid(const DevParams &p) {
+RdfGrid rdf_grid(const DevParams &p, int S) {
     RdfGrid g{};
     const float range = sqrtf(p.rangeRdf2);
     if (!(range > 0.f) || range >= 0.5f * p.Lx || range >= 0.5f * p.Ly || range >= 0.5f * p.Lz) return g;
-    // 1e-4 wider than half the range: the float rounding of a cell index (~1e-7 of the box) can then
-    // never put a pair within range three cells apart
-    const float w = 0.5f * range * 1.0001f;
+    S = std::min(std::max(S, 2), 6);
+    // 1e-4 wider than range/S: the float rounding of a cell index (~1e-7 of the box) can then never
+    // put a pair within range S + 1 cells apart
+    const float w = range / (float)S * 1.0001f;
     g.nx = std::max(1, (int)(p.Lx / w));
     g.ny = std::max(1, (int)(p.Ly / w));
     g.nz = std::max(1, (int)(p.Lz / w));
@@ -1203,7 +1221,11 @@ RdfGrid rdf_grid(const DevParams &p) {
     g.iwx = (float)g.nx / p.Lx;
     g.iwy = (float)g.ny / p.Ly;
     g.iwz = (float)g.nz / p.Lz;
-    g.sx = g.sy = g.sz = 2;
+    // culling uses the nominal width range/S (<= L/n) and a range 0.1 % long, far above the ~1e-7
+    // of the box by which a rounded cell index can misplace an atom
+    g.wx = g.wy = g.wz = range / (float)S;
+    g.cull2 = range * range * 1.002f;
+    g.sx = g.sy = g.sz = S;
     g.ncell = g.nx * g.ny * g.nz;
     return g;
 }

@@ -82,7 +82,9 @@ struct DevState {
 struct RdfGrid {
     int nx, ny, nz, ncell;
     float iwx, iwy, iwz;
-    int sx, sy, sz;
+    float wx, wy, wz;  // cell widths L/n
+    float cull2;       // a cell whose box lies at least this far (squared) from the atom's cell is skipped
+    int sx, sy, sz;    // stencil half-widths in cells
 };
 
 // Device buffers of the centre-of-mass diffusion (sgk_diffusion_enable): nBuff staggered buffers of
