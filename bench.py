@@ -164,9 +164,10 @@ def batch_workload(nrep=64, seed=2024):
 
 
 def bench_batch(steps, warmup, nrep, tune=True, sequential=True):
-    """Atom-steps/s of the whole ensemble batched (one launch per stage), with lanes per metal atom
-    tuned per replica-size bucket after the warm-up (the table is reported), and, for comparison,
-    of the same replicas advanced one after another, each in its own context."""
+    """Atom-steps/s of the whole ensemble batched (one launch per stage, list rebuilds synchronised
+    across replicas), with lanes per metal atom tuned per replica-size bucket after the warm-up (the
+    table is reported), and, for comparison, of the same replicas advanced one after another, each in
+    its own context."""
     import surfacegrowthconsole_amd as sg
 
     systems = batch_workload(nrep)
@@ -174,8 +175,12 @@ def bench_batch(steps, warmup, nrep, tune=True, sequential=True):
     log(f"[bench] batch: {nrep} replicas, {natoms} atoms")
     res = {"workload": f"{nrep} uneven replicas (55-2000 Ni, 2016-16128 C, 300-1200 K)", "replicas": nrep,
            "atoms": natoms, "steps": steps}
+    res["sync_rebuilds"] = 1
     for prec in ("mixed", "fp32"):
-        with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, records_capacity=500) as bt:
+        # sync_rebuilds: a step that rebuilds one replica's lists rebuilds all (the list kernels then
+        # run on about one step in five instead of on nearly every step)
+        with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, records_capacity=500,
+                      sync_rebuilds=1) as bt:
             for k, s in enumerate(systems):
                 bt.upload(k, s.r, s.v, s.a)
                 bt.set_run_state(k, 0, 1, 0.0)

@@ -74,6 +74,11 @@ typedef struct sgk_config {
                              over a grid of cells rangeRdf/2 wide; 2-6 the same over cells rangeRdf/k
                              wide, cells beyond range culled (same histogram; on the 100,000-atom
                              cluster 2 is the fastest).  No counterpart in the reference. */
+    int sync_rebuilds;    /* batched engine: when the skin rule or the cadence of any replica calls for a
+                             list rebuild, every replica rebuilds at that step, so the rebuild launches
+                             run on fewer steps.  A replica then equals the replica run alone with the
+                             same rebuild steps.  0 (default): each replica decides alone.  No
+                             counterpart in the reference. */
 } sgk_config;
 
 /* One record per step, written on the device by the reduction stage.  Sums are over atoms with

@@ -113,7 +113,7 @@ class SgkConfig(C.Structure):
         ("close_capacity", _i), ("close_margin", _f), ("metal_lanes", _i), ("block_size", _i),
         ("keep_full_nl", _i), ("rebuild_every", _i), ("records_capacity", _i), ("use_graphs", _i),
         ("metal_thermostat_all", _i), ("pair_fp64", _i), ("xcd_map", _i), ("bin_group", _i),
-        ("nl_scheme", _i), ("rdf_complete", _i),
+        ("nl_scheme", _i), ("rdf_complete", _i), ("sync_rebuilds", _i),
     ]
 
 

@@ -505,6 +505,16 @@ void launch_batch_lf1(sgk_batch *b, bool kicked) {
         hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked);
 }
 
+// sync_rebuilds: when any replica rebuilds at the next step, all do.  One workgroup, after the
+// replicas' finalize of the step.
+__global__ __launch_bounds__(256) void kb_sync_rebuild(DevState *st, int nrep) {
+    int any = 0;
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) any |= st[r].nebrNow;
+    any = __syncthreads_or(any);
+    if (any)
+        for (int r = threadIdx.x; r < nrep; r += blockDim.x) st[r].nebrNow = 1;
+}
+
 void launch_batch_finalize(sgk_batch *b) {
     if (b->mixed) {
         hipLaunchKernelGGL(kb_reduce1<true>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
@@ -513,6 +523,8 @@ void launch_batch_finalize(sgk_batch *b) {
         hipLaunchKernelGGL(kb_reduce1<false>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
         hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
     }
+    if (b->cfg.sync_rebuilds && b->nrep > 1)
+        hipLaunchKernelGGL(kb_sync_rebuild, dim3(1), dim3(256), 0, b->stream, b->state, b->nrep);
 }
 
 void launch_batch_thermostat(sgk_batch *b) {

@@ -115,3 +115,41 @@ def test_batch_autotune_per_size_bucket(gpu):
         scale = np.abs(b).max()
         assert np.abs(a - b).max() <= 1e-12 * scale
     assert all(np.isfinite(x["totEnergy"]) and x["errors"] == 0 for rr in recs for x in rr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_batch_sync_rebuilds_equals_replicas_with_the_same_rebuild_steps(gpu, prec):
+    """sync_rebuilds: a step rebuilds every replica's lists when any replica's own rule asked for it
+    (the records keep each replica's own decision).  Each replica then equals the replica run alone
+    with its list rebuilt at those same steps, bit for bit."""
+    systems = replicas()
+    steps = 60
+    print(f"[batch] {prec}: sync_rebuilds", flush=True)
+    with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=0, metal_lanes=4, xcd_map=0,
+                  sync_rebuilds=1) as bt:
+        for k, s in enumerate(systems):
+            bt.upload(k, s.r, s.v, s.a)
+            bt.set_run_state(k, 0, 1, 0.0)
+        recs = bt.run(steps)
+        out = [bt.download(k) for k in range(len(systems))]
+    rebuilt = np.array([[x["rebuilt"] for x in r] for r in recs])
+    own = np.array([[x["nebrNow"] for x in r] for r in recs])
+    assert (rebuilt == rebuilt[0]).all()
+    assert rebuilt[0, 0] == 1 and np.array_equal(rebuilt[0, 1:], own[:, :-1].any(0))
+    assert 1 < rebuilt[0].sum() and (own[:, :-1].sum(1) < rebuilt[0, 1:].sum()).any()  # schedules differ
+    for k, s in enumerate(systems):
+        with sg.Engine(s.params, precision=prec, rebuild_every=0, metal_lanes=4, xcd_map=0,
+                       records_capacity=1) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            ref = []
+            for step in range(steps):
+                if step > 0 and rebuilt[0, step]:
+                    eng.set_run_state(step, 1, ref[-1]["dispHi"])
+                ref += eng.run(1)
+            r, v, _ = eng.download()
+        assert [x["rebuilt"] for x in ref] == rebuilt[0].tolist(), k
+        assert np.array_equal(out[k][0][:, :3], r[:, :3]), k
+        assert np.array_equal(out[k][1], v), k
+        assert [x["totEnergy"] for x in recs[k]] == [x["totEnergy"] for x in ref], k
