@@ -468,7 +468,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", default="large", choices=["small", "medium", "large", "icosa"])
+    ap.add_argument("--config", default="large",
+                    choices=["small", "medium", "large", "icosa", "shard8_sheet", "shard8"])
     ap.add_argument("--variant", default="config", choices=["config", "reference"])
     ap.add_argument("--chunk", type=int, default=500)
     ap.add_argument("--rdf-every", type=int, default=-1,

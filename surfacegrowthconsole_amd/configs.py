@@ -29,6 +29,13 @@ CONFIGS = {
     # large: 1,003,520 C + 100,000 Ni sphere (r ~ 6.4 nm), 1,000 K, RDF 400 bins to 12 A
     "large": dict(ux=160, uy=196, n_metal=100000, temperature=1000.0, step_limit=2000, step_rdf=50,
                   rangeRdfAngstrom=12.0, sizeHistRdf=400),
+    # one x-slab of the large workload cut 8 ways (20 of its 160 sheet columns, 125,440 C), alone and
+    # with an eighth of its metal as a 12,500-atom cluster (137,940 atoms, an eighth of the whole):
+    # the shards a multi-GPU run of the large workload would hand one GPU, timed on one GPU (no halo)
+    "shard8_sheet": dict(ux=20, uy=196, n_metal=0, temperature=1000.0, step_limit=2000, step_rdf=50,
+                         rangeRdfAngstrom=12.0, sizeHistRdf=400),
+    "shard8": dict(ux=20, uy=196, n_metal=12500, temperature=1000.0, step_limit=2000, step_rdf=50,
+                   rangeRdfAngstrom=12.0, sizeHistRdf=400),
 }
 
 _VARIANTS = {
