@@ -180,9 +180,11 @@ def test_list_schemes_same_trajectory(gpu, name, prec):
         assert np.array_equal(x, y)
 
 
-def check_carbon_lists(eng, p, r, nn, nbl):
+def check_carbon_lists(eng, p, r, nn, nbl, lj_carbons_positions_only=False):
     """Carbon lists = the metal entries of the reference list, and the entries closer than the close
-    radius, each with its list position, in list order."""
+    radius, each with its list position, in list order.  lj_carbons_positions_only: compare the close
+    entries' list positions only for carbons with LJ partners (the only place positions are read);
+    then every carbon with LJ partners is checked, beside 400 drawn at random."""
     cl = eng.carbon_lists()
     N, nMe = p.nMol, p.nMolMe
     nbl = nbl.reshape(-1, N)
@@ -191,7 +193,10 @@ def check_carbon_lists(eng, p, r, nn, nbl):
     close2 = np.float32(close_r * close_r)
     L = np.array([p.region.x, p.region.y, p.region.z], np.float32)
     rng = np.random.default_rng(0)
-    for i in rng.choice(np.arange(nMe, N), size=min(400, N - nMe), replace=False):
+    picks = rng.choice(np.arange(nMe, N), size=min(400, N - nMe), replace=False)
+    if lj_carbons_positions_only:
+        picks = np.union1d(picks, nMe + np.nonzero(cl["n_lj"])[0])
+    for i in picks:
         ic = i - nMe
         ks = nbl[: nn[i], i]
         pos = np.arange(nn[i])
@@ -203,7 +208,39 @@ def check_carbon_lists(eng, p, r, nn, nbl):
         rr = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
         close = rr < close2
         assert np.array_equal(cl["close_idx"][ic, : cl["n_close"][ic]], ks[close])
-        assert np.array_equal(cl["close_pos"][ic, : cl["n_close"][ic]], pos[close])
+        if not lj_carbons_positions_only or cl["n_lj"][ic] > 0:
+            assert np.array_equal(cl["close_pos"][ic, : cl["n_close"][ic]], pos[close])
+
+
+@pytest.mark.parametrize("name", ["small", "icosa", "medium"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_carbon_lists_without_the_reference_layout(gpu, name, prec):
+    """The engines of the step loop keep no reference-layout list (keep_full_nl off): their carbon
+    lists, built in the list-build order, equal the reference list's entries in order with their
+    list positions, and their forces equal those of an engine that also keeps the reference layout,
+    bit for bit."""
+    r, _, _ = thermalised(name)
+    s = system(name)
+    p = s.params
+    rin = r if prec == "mixed" else np.asarray(r, np.float32)
+    o = oracle.Oracle(p, "f32")
+    r32 = np.asarray(r, np.float32)
+    err, ocell, ocnt, oslots = o.bin(r32)
+    err2, onn, onbl = o.nebr(r32, ocnt, oslots)
+    assert err == 0 and err2 == 0
+    out = []
+    for keep in (False, True):
+        print(f"[parity] carbon lists {name} {prec} keep_full_nl={keep}", flush=True)
+        with sg.Engine(p, precision=prec, keep_full_nl=keep, nl_scheme=3) as eng:
+            eng.upload(rin)
+            eng.build_nl_order()
+            eng.bin()
+            eng.build_nebr()
+            check_carbon_lists(eng, p, r32, onn, onbl)
+            eng.compute_forces()
+            out.append(eng.forces())
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("name", ["small", "icosa", "medium"])
