@@ -362,26 +362,32 @@ __device__ __forceinline__ double rcpR(double x) { return rcp64(x); }
 __device__ __forceinline__ float sqrtR(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrtR(double x) { return sqrt64(x); }
 
+// Graphene-law accumulation in the storage precision: on a hot sheet a carbon's terms reach ~1e3 and
+// cancel to a force of ~1e2, so the mixed form sums them in FP64 (the terms themselves stay FP32);
+// the plain form sums in FP32, as the reference.
+__device__ __forceinline__ void gacc(float &a, float t) { a += t; }
+__device__ __forceinline__ void gacc(double &a, float t) { a += (double)t; }
+
 // Neighbour-angle terms of bond (i,k) with k's bonded neighbour m (SurfaceGrowth.cu:953-978).
 // rik: r_i - r_k, aik = |rik|; e: r_m - r_k, rr2 = |e|^2 (the reference divides by rr2 where |e|
 // is meant; kept).
 template <class R>
 __device__ __forceinline__ void neighbour_angle(const DevParams &p, const V3<R> &rik, R aik, const V3<R> &e, R rr2,
-                                                float mz_rel, float &fx, float &fy, float &fz) {
+                                                float mz_rel, R &fx, R &fy, R &fz) {
     const R c = (rik.x * e.x + rik.y * e.y + rik.z * e.z) * rcpR(aik * rr2);
     float s;
     const float dth = AngleDev<R>::eval(c, p, s);
     const float tmp = p.mu_theta * dth * rcpf(s);
     const float fa = (float)aik, frr = (float)rr2;
     float fc = tmp * rcpf(fa * frr);
-    fx += fc * (float)e.x;
-    fy += fc * (float)e.y;
-    fz += fc * (float)e.z;
+    gacc(fx, fc * (float)e.x);
+    gacc(fy, fc * (float)e.y);
+    gacc(fz, fc * (float)e.z);
     fc = -tmp * (float)c * rcpf(fa * fa);
-    fx += fc * (float)rik.x;
-    fy += fc * (float)rik.y;
-    fz += fc * (float)rik.z;
-    fz += -p.mu_p * mz_rel * (1.f / 9.f);
+    gacc(fx, fc * (float)rik.x);
+    gacc(fy, fc * (float)rik.y);
+    gacc(fz, fc * (float)rik.z);
+    gacc(fz, -p.mu_p * mz_rel * (1.f / 9.f));
 }
 
 // ComputeForcesK carbon branch (SurfaceGrowth.cu:899-1062, 1083-1088), one thread per carbon.
@@ -400,7 +406,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
     const int i = p.nMolMe + ic;
     const P4 Ai = pos[i];
     const R z0 = (R)p.z_0;
-    float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
+    R fx = R(0), fy = R(0), fz = R(0), u = R(0);
     V3<R> rnn[3];
     R absR[3];
     float dZ[3];
@@ -462,16 +468,16 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         for (int b = 0; b < 3; ++b) {
             const float dr = (float)(absR[b] - R(1));
             const float fc = -p.mu_r * dr * rcpf((float)absR[b]);
-            fx += fc * (float)rnn[b].x;
-            fy += fc * (float)rnn[b].y;
-            fz += fc * (float)rnn[b].z;
-            u += p.mu_r * dr * dr;
+            gacc(fx, fc * (float)rnn[b].x);
+            gacc(fy, fc * (float)rnn[b].y);
+            gacc(fz, fc * (float)rnn[b].z);
+            gacc(u, p.mu_r * dr * dr);
         }
         const float zi = (float)(Ai.z - z0);
         float dz = 2.f * zi - (dZ[0] + dZ[1] + dZ[2]);
-        fz += -0.666666667f * p.mu_p * dz;
+        gacc(fz, -0.666666667f * p.mu_p * dz);
         dz = zi - 0.33333333f * (dZ[0] + dZ[1] + dZ[2]);
-        u += p.mu_p * dz * dz;
+        gacc(u, p.mu_p * dz * dz);
 #pragma unroll
         for (int qq = 0; qq < 3; ++qq) {
             const int a = qq, b = qq == 2 ? 0 : qq + 1;
@@ -484,10 +490,10 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
             const float inv = ira * irb;
             const float fcb = tmp * (1.f - ra * fcos * irb) * inv;
             const float fca = tmp * (1.f - rb * fcos * ira) * inv;
-            fx += fcb * (float)rnn[b].x + fca * (float)rnn[a].x;
-            fy += fcb * (float)rnn[b].y + fca * (float)rnn[a].y;
-            fz += fcb * (float)rnn[b].z + fca * (float)rnn[a].z;
-            u += p.mu_theta * dth * dth;
+            gacc(fx, fcb * (float)rnn[b].x + fca * (float)rnn[a].x);
+            gacc(fy, fcb * (float)rnn[b].y + fca * (float)rnn[a].y);
+            gacc(fz, fcb * (float)rnn[b].z + fca * (float)rnn[a].z);
+            gacc(u, p.mu_theta * dth * dth);
         }
     }
     // LJ with metal.  The reference reuses its neighbour-loop counter for the bond loop
