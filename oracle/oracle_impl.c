@@ -22,6 +22,7 @@
 #endif
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -255,6 +256,16 @@ static inline REAL eam_dF(const orc_params *p, REAL rho) { return eam_dF_br(p, r
  * bits of every density, force and adhesion pass, the branches of every force pass); in replay mode it takes them from
  * the tape instead, so a run with perturbed parameters follows the recorded run's decisions and its
  * finite differences are those of the map the adjoint differentiates.  Test infrastructure only. */
+/* Allocation for the decision tape and the cutoff bits: test infrastructure, so running out of
+ * memory ends the process with a message. */
+static inline void *orc_need(void *q) {
+    if (!q) {
+        fprintf(stderr, "oracle: out of memory\n");
+        abort();
+    }
+    return q;
+}
+
 #if defined(ORC_F64)
 static struct {
     int mode; /* 0 off, 1 record, 2 replay */
@@ -267,7 +278,7 @@ static inline void tape_push(const int *x, size_t n) {
     if (g_tape.len + n > g_tape.cap) {
         size_t c = g_tape.cap ? 2 * g_tape.cap : 1 << 20;
         while (c < g_tape.len + n) c *= 2;
-        g_tape.v = (int *)realloc(g_tape.v, c * sizeof(int));
+        g_tape.v = (int *)orc_need(realloc(g_tape.v, c * sizeof(int)));
         g_tape.cap = c;
     }
     memcpy(g_tape.v + g_tape.len, x, n * sizeof(int));
@@ -318,12 +329,12 @@ static unsigned char *FN(cut_tape)(const orc_params *p, const REAL *pos4, const 
     if (!tape_mode()) return NULL;
     const int nMe = p->nMolMe;
     const REAL Lx = p->region[0], Ly = p->region[1], Lz = p->region[2];
-    size_t *o = (size_t *)malloc(sizeof(size_t) * ((size_t)nRows + 1));
+    size_t *o = (size_t *)orc_need(malloc(sizeof(size_t) * ((size_t)nRows + 1)));
     o[0] = 0;
     for (int i = 0; i < nRows; ++i) o[i + 1] = o[i] + (size_t)NN[i];
     const size_t n = o[nRows], nw = (n + 31) / 32;
-    unsigned char *cut = (unsigned char *)malloc(n ? n : 1);
-    int *w = (int *)calloc(nw ? nw : 1, sizeof(int));
+    unsigned char *cut = (unsigned char *)orc_need(malloc(n ? n : 1));
+    int *w = (int *)orc_need(calloc(nw ? nw : 1, sizeof(int)));
     if (tape_mode() == 2) {
         tape_pull(w, nw);
         for (size_t q = 0; q < n; ++q) cut[q] = (unsigned char)(((unsigned)w[q / 32] >> (q % 32)) & 1u);
@@ -394,7 +405,7 @@ static void FN(forces_impl)(const orc_params *p, REAL *pos4, const int *NN, cons
                th0 = p->theta_0;
     int *br = NULL; /* F(rho) branches from the decision tape (record or replay), else from rho */
     if (tape_mode() && nMe) {
-        br = (int *)malloc(sizeof(int) * (size_t)nMe);
+        br = (int *)orc_need(malloc(sizeof(int) * (size_t)nMe));
         if (tape_mode() == 2) {
             tape_pull(br, (size_t)nMe);
         } else {
