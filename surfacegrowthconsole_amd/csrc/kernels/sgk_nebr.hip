@@ -172,8 +172,10 @@ __global__ __launch_bounds__(256) void k_sortcells(DevParams p, const typename P
 template <bool M>
 __global__ __launch_bounds__(256) void kb_sortcells(BatchArgs a) {
     const int2 e = a.atomTab[blockIdx.x];
+    DevState *st = a.state + e.x;
+    if (!st->nebrNow) return;  // tested before the replica's table entry is read: most steps rebuild nothing
     const BatchRep &B = a.reps[e.x];
-    sortcells_body<M>(B.p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), a.state + e.x, 0);
+    sortcells_body<M>(B.p, e.y, (const typename Prec<M>::P4 *)a.pos + B.atomOff, rep_cells(a, B), st, 0);
 }
 
 // BuildNebrListK (SurfaceGrowth.cu:612-711), one thread per atom over cells whose slots are sorted

@@ -182,24 +182,38 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(DevParams p,
     finalize_body<typename Prec<M>::A>(p, part, nParts, st, records, recCap, mode);
 }
 
-// Batched: the same two stages per replica (one workgroup per replica each), so a replica's
-// reduction order is that of the replica run alone.
+// Batched: both stages in one workgroup per replica - the waves reduce the chunks (the calls of
+// k_reduce1, in the same order), then the workgroup finalizes - so a replica's reduction order is
+// that of the replica run alone.
+// syncCount != nullptr (synchronised rebuilds): the workgroup that finishes last ORs the replicas'
+// rebuild flags into every replica and puts the count back to 0 for the next launch.  The records
+// keep each replica's own decision.
 template <bool M>
-__global__ __launch_bounds__(256) void kb_reduce1(BatchArgs a) {
+__global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a, int nrep, int *syncCount) {
     using A = typename Prec<M>::A;
     const BatchRep &B = a.reps[blockIdx.x];
     const Partial<A> *part = (const Partial<A> *)a.partials + B.partOff;
     Partial<A> *part2 = (Partial<A> *)a.partials + B.part2Off;
-    for (int j = threadIdx.x >> 6; j < B.nChunks; j += 4)
+    for (int j = threadIdx.x >> 6; j < B.nChunks; j += kFinalizeThreads / 64)
         reduce_chunk(part + j * kPartChunk, min(kPartChunk, B.nbForce - j * kPartChunk), part2 + j);
-}
-
-template <bool M>
-__global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a) {
-    using A = typename Prec<M>::A;
-    const BatchRep &B = a.reps[blockIdx.x];
-    finalize_body<A>(B.p, (const Partial<A> *)a.partials + B.part2Off, B.nChunks, a.state + blockIdx.x,
-                     a.records + (size_t)blockIdx.x * a.recCap, a.recCap, 0);
+    __syncthreads();
+    finalize_body<A>(B.p, part2, B.nChunks, a.state + blockIdx.x, a.records + (size_t)blockIdx.x * a.recCap,
+                     a.recCap, 0);
+    if (!syncCount) return;
+    __shared__ int last;
+    __threadfence();  // this replica's state reaches device scope before the count
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(syncCount, 1) == nrep - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    int any = 0;
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x)
+        any |= __hip_atomic_load(&a.state[r].nebrNow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    any = __syncthreads_or(any);
+    if (any)
+        for (int r = threadIdx.x; r < nrep; r += blockDim.x) a.state[r].nebrNow = 1;
+    if (threadIdx.x == 0) *syncCount = 0;
 }
 
 // ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485), kinetic energy of all atoms.
@@ -507,24 +521,12 @@ void launch_batch_lf1(sgk_batch *b, bool kicked) {
 
 // sync_rebuilds: when any replica rebuilds at the next step, all do.  One workgroup, after the
 // replicas' finalize of the step.
-__global__ __launch_bounds__(256) void kb_sync_rebuild(DevState *st, int nrep) {
-    int any = 0;
-    for (int r = threadIdx.x; r < nrep; r += blockDim.x) any |= st[r].nebrNow;
-    any = __syncthreads_or(any);
-    if (any)
-        for (int r = threadIdx.x; r < nrep; r += blockDim.x) st[r].nebrNow = 1;
-}
-
 void launch_batch_finalize(sgk_batch *b) {
-    if (b->mixed) {
-        hipLaunchKernelGGL(kb_reduce1<true>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
-        hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
-    } else {
-        hipLaunchKernelGGL(kb_reduce1<false>, dim3(b->nrep), dim3(256), 0, b->stream, b->args());
-        hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
-    }
-    if (b->cfg.sync_rebuilds && b->nrep > 1)
-        hipLaunchKernelGGL(kb_sync_rebuild, dim3(1), dim3(256), 0, b->stream, b->state, b->nrep);
+    int *sync = b->cfg.sync_rebuilds && b->nrep > 1 ? b->syncCount : nullptr;
+    if (b->mixed)
+        hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args(), b->nrep, sync);
+    else
+        hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args(), b->nrep, sync);
 }
 
 void launch_batch_thermostat(sgk_batch *b) {
