@@ -254,7 +254,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
     for (int r = 0; r < nrep; ++r) b->repLanes.push_back(fixedG ? G : default_lanes(params[r].nMolMe));
     (void)sumMe;
     // replica table and block tables; per-replica workgroup decomposition as in sgk_ctx
-    std::vector<int2> atomTab, metalTab, nlTab;
+    std::vector<int2> atomTab, metalTab, nlTab, nlTabC;
     long long atomOff = 0, metalOff = 0, carbonOff = 0, cellOff = 0, ordOff = 0;
     for (int r = 0; r < nrep; ++r) {
         const SimParams &s = params[r];
@@ -272,10 +272,11 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         const int nbC = (R.p.nC + 255) / 256;
         for (int k = 0; k < nbAtom; ++k) atomTab.push_back(make_int2(r, k));
         for (int k = 0; k < nbMetal; ++k) metalTab.push_back(make_int2(r, k));
-        // cell-ordered list build: kNlMetalLanes (4) lanes per metal atom, then one lane per carbon
+        // cell-ordered list build: kNlMetalLanes (4) lanes per metal atom, then one lane per carbon;
+        // every replica's metal workgroups (the long walks) are dispatched before any carbon one
         R.nbNlM = (int)(((long long)s.nMolMe * 4 + 255) / 256);
-        const int nbNl = R.nbNlM + nbC;
-        for (int k = 0; k < nbNl; ++k) nlTab.push_back(make_int2(r, k));
+        for (int k = 0; k < R.nbNlM; ++k) nlTab.push_back(make_int2(r, k));
+        for (int k = 0; k < nbC; ++k) nlTabC.push_back(make_int2(r, R.nbNlM + k));
         R.ordOff = (int)ordOff;
         ordOff += 2LL * R.p.ncell + 2;
         b->reps.push_back(R);
@@ -284,6 +285,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         carbonOff += R.p.nC;
         cellOff += R.p.ncell;
     }
+    nlTab.insert(nlTab.end(), nlTabC.begin(), nlTabC.end());
     const std::vector<int2> forceTab = force_layout(b);
     size_t tabCap = 0, partCap = 0;
     worst_layout(b, tabCap, partCap);

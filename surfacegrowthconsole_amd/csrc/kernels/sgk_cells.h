@@ -33,11 +33,11 @@ __device__ __forceinline__ void bin_atom(const DevParams &p, int n, float3 q, co
     const int c = (cz * p.cy + cy) * p.cx + cx;
     const int b = st->cellbuf ^ 1;
     if (c < 0 || c >= p.ncell) {  // no reference block has this index: the atom is not binned
-        bufs.cellOf[b][n] = -1;
+        bufs.cellOfB(b)[n] = -1;
         atomicOr(&st->errors, SGK_FLAG_UNBINNED);
         return;
     }
-    bufs.cellOf[b][n] = c;
+    bufs.cellOfB(b)[n] = c;
     // Lanes of a wave that fall in the same cell (consecutive atoms are spatial neighbours) take
     // their slots with one atomic: group the active lanes by cell, one pass per distinct cell,
     // then one atomicAdd per group leader.  Device-scope atomics on one address serialise, so
@@ -57,12 +57,12 @@ __device__ __forceinline__ void bin_atom(const DevParams &p, int n, float3 q, co
         todo &= ~m;
     }
     int base = 0;
-    if (lane == leader) base = atomicAdd(&bufs.cnt[b][c], __popcll(grp));
+    if (lane == leader) base = atomicAdd(&bufs.cntB(b)[c], __popcll(grp));
     base = __shfl(base, leader, 64);
     const int slot = base + __popcll(grp & below);
     bufs.slotOf[n] = slot;
     if (slot < p.cellCap)
-        bufs.slots[b][(size_t)c * p.cellCap + slot] = n;
+        bufs.slotsB(b)[(size_t)c * p.cellCap + slot] = n;
     else
         atomicOr(&st->errors, SGK_FLAG_CELL_OVERFLOW);
 }

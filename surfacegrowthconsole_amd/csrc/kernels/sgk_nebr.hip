@@ -114,7 +114,7 @@ __device__ __forceinline__ void sort_cells(const DevParams &p, const CellBufs &b
         const int c = __shfl(cn, src < 0 ? 0 : src, 64);
         const int cntSrc = __shfl(ocnt, src < 0 ? 0 : src, 64);
         const int cnt = src < 0 ? 0 : cntSrc;
-        int *s = bufs.slots[b] + (size_t)(src < 0 ? 0 : c) * p.cellCap;
+        int *s = bufs.slotsB(b) + (size_t)(src < 0 ? 0 : c) * p.cellCap;
         int v = idx < cnt ? s[idx] : 0x7fffffff;
 #pragma unroll
         for (int k = 2; k <= W; k <<= 1) {
@@ -150,12 +150,12 @@ __device__ __forceinline__ void sortcells_body(const DevParams &p, int blk, cons
     const bool in = n < p.nMol;  // every lane stays for the wave-wide sort
     int cn = -1;
     if (in) {
-        const int co = bufs.cellOf[old][n];
-        if (co >= 0) bufs.cnt[old][co] = 0;
-        cn = bufs.cellOf[b][n];
+        const int co = bufs.cellOfB(old)[n];
+        if (co >= 0) bufs.cntB(old)[co] = 0;
+        cn = bufs.cellOfB(b)[n];
     }
     const bool owner = in && cn >= 0 && bufs.slotOf[n] == 0;
-    const int ocnt = owner ? min(bufs.cnt[b][cn], p.cellCap) : 0;
+    const int ocnt = owner ? min(bufs.cntB(b)[cn], p.cellCap) : 0;
     // Cells of up to 16 atoms (most of them) are sorted four at a time, 16 lanes each; up to 32, two
     // at a time; larger ones with the whole wave.
     sort_cells<16>(p, bufs, b, pos, cn, ocnt, __ballot(owner && ocnt <= 16));
@@ -244,11 +244,11 @@ __device__ __forceinline__ void nl_body(const DevParams &p, int blk, const typen
     const int i = blk * blockDim.x + threadIdx.x;
     if (i >= p.nMol) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
-    const int *cnt = bufs.cnt[b];
+    const int *cnt = bufs.cntB(b);
     const float4 *cpos = bufs.cellPos;
     const bool metal = i < p.nMolMe;
     const int ic = i - p.nMolMe;
-    const int c = bufs.cellOf[b][i];
+    const int c = bufs.cellOfB(b)[i];
     int nn = 0, nljv = 0, nclv = 0, err = 0;
     if (c >= 0) {
         const int ccz = c / (p.cx * p.cy);
@@ -351,7 +351,7 @@ __device__ __forceinline__ void nlt_body(const DevParams &p, int blk, const type
     const int i = blk * blockDim.x + threadIdx.x;
     const bool in = i < p.nMol;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
-    const int c = in ? bufs.cellOf[b][i] : -1;
+    const int c = in ? bufs.cellOfB(b)[i] : -1;
     int ccx = 0, ccy = 0, ccz = 0;
     if (c >= 0) {
         ccz = c / (p.cx * p.cy);
@@ -379,7 +379,7 @@ __device__ __forceinline__ void nlt_body(const DevParams &p, int blk, const type
     const int x0 = sLo[0] - 1, y0 = sLo[1] - 1, z0 = sLo[2] - 1;  // box origin (grid coordinates)
     const int BX = sHi[0] - sLo[0] + 3, BY = sHi[1] - sLo[1] + 3, BZ = sHi[2] - sLo[2] + 3;
     const long long Hl = (long long)BX * BY * BZ;
-    const int *cnt = bufs.cnt[b];
+    const int *cnt = bufs.cntB(b);
     // box cell h -> grid cell, with the reference's single wrap per axis (stencil_cell)
     auto grid_cell = [&](int h) {
         const int hz = h / (BX * BY), hy = (h / BX) % BY, hx = h % BX;
@@ -456,12 +456,12 @@ __device__ __forceinline__ void nlw_body(const DevParams &p, int blk, const Cell
     const int n = blk * blockDim.x + threadIdx.x;
     const bool in = n < p.nMol;  // every lane stays: the work below is wave-wide
     const int cap = p.cellCap;
-    const int *cnt = bufs.cnt[b];
-    const int *slots = bufs.slots[b];
+    const int *cnt = bufs.cntB(b);
+    const int *slots = bufs.slotsB(b);
     const float4 *cpos = bufs.cellPos;
     int cn = -1;
     if (in) {
-        cn = bufs.cellOf[b][n];
+        cn = bufs.cellOfB(b)[n];
         if (cn < 0) {  // not binned: empty lists
             if (n < p.nMolMe)
                 nnM[n] = 0;
@@ -650,7 +650,7 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
     const int g = gt / G, l = gt % G;
     const bool active = g < p.nMolMe;
     const int i = active && order ? order[p.nC + g] : g;  // metal atoms by cell (index order without one)
-    const int c = active ? bufs.cellOf[b][i] : -1;
+    const int c = active ? bufs.cellOfB(b)[i] : -1;
     const int gshift = (threadIdx.x & 63) & ~(G - 1);
     const unsigned gmask = (1u << G) - 1u, below = (1u << l) - 1u;
     int nn = 0;
@@ -660,7 +660,7 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
         const int ccx = c - (ccz * p.cy + ccy) * p.cx;
         const float3 A = f3(pos[i]);
         const float rrN = p.rrNebr;
-        const int *cnt = bufs.cnt[b];
+        const int *cnt = bufs.cntB(b);
         const bool wrapSelf = p.cx < 3 || p.cy < 3 || p.cz < 3;
         // one column (offsets 3q, 3q+1, 3q+2: same x and y shift) at a time, its cells back to back
         const int cap = p.cellCap;
@@ -822,12 +822,12 @@ __device__ __forceinline__ void nlc_body(const DevParams &p, int blk, const type
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= p.nC) return;
     const int i = order ? order[t] : p.nMolMe + t;
-    const int c = bufs.cellOf[b][i];
+    const int c = bufs.cellOfB(b)[i];
     if (c < 0) {  // not binned: empty lists
         nl_finish(p, i, false, i - p.nMolMe, 0, 0, 0, 0, nullptr, nlj, ncl, fullNN, st);
         return;
     }
-    nlc_walk<KEEP>(p, i, c, f3(pos[i]), bufs.cnt[b], bufs.cellPos, st, lj, nlj, cl, ncl, fullNN,
+    nlc_walk<KEEP>(p, i, c, f3(pos[i]), bufs.cntB(b), bufs.cellPos, st, lj, nlj, cl, ncl, fullNN,
                    fullNBL);
 }
 
@@ -1039,12 +1039,12 @@ __device__ __forceinline__ void rdf_body(const DevParams &p, int blk, const type
     for (int b = threadIdx.x; b < p.sizeHistRdf; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     const int i = blk * blockDim.x + threadIdx.x;
-    const int c = i < p.nMolMe ? bufs.cellOf[st->cellbuf][i] : -1;
+    const int c = i < p.nMolMe ? bufs.cellOfB(st->cellbuf)[i] : -1;
     int edge = 0;
     if (c >= 0) {
         const int cb = st->cellbuf;
-        const int *cnt = bufs.cnt[cb];
-        const int *slots = bufs.slots[cb];
+        const int *cnt = bufs.cntB(cb);
+        const int *slots = bufs.slotsB(cb);
         const int ccz = c / (p.cx * p.cy);
         const int ccy = (c - ccz * p.cx * p.cy) / p.cx;
         const int ccx = c - (ccz * p.cy + ccy) * p.cx;

@@ -51,6 +51,11 @@ struct CellBufs {
     int *cellOf[2];
     int *slotOf;
     float4 *cellPos;
+    // Buffer k (0 or 1) taken by a select, not by indexing: a kernel that builds the struct in
+    // registers (the batched ones, per replica) keeps it there instead of copying it to memory.
+    __host__ __device__ int *cntB(int k) const { return k ? cnt[1] : cnt[0]; }
+    __host__ __device__ int *slotsB(int k) const { return k ? slots[1] : slots[0]; }
+    __host__ __device__ int *cellOfB(int k) const { return k ? cellOf[1] : cellOf[0]; }
 };
 
 // Per-block partial reductions written by the force kernels (or the props kernel).
