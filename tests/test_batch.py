@@ -153,3 +153,33 @@ def test_batch_sync_rebuilds_equals_replicas_with_the_same_rebuild_steps(gpu, pr
         assert np.array_equal(out[k][0][:, :3], r[:, :3]), k
         assert np.array_equal(out[k][1], v), k
         assert [x["totEnergy"] for x in recs[k]] == [x["totEnergy"] for x in ref], k
+
+
+@pytest.mark.gpu
+def test_batch_sync_rebuilds_over_more_replicas_than_a_workgroup_has_lanes(gpu):
+    """The OR of the rebuild flags runs in the last replica workgroup of the closing kernel, 256
+    threads striding over the replicas.  With 300 replicas - the first 256 cold, the last 44 hot -
+    steps occur where only replicas past the first stride asked for a rebuild, and every replica must
+    still rebuild on exactly the steps after one where any replica asked."""
+    n_cold, n_hot = 256, 44
+    systems = [sg.config_system("small", "config", ux=3, uy=3, n_metal=13,
+                                temperature=300.0 if k < n_cold else 1500.0, seed=100 + k)
+               for k in range(n_cold + n_hot)]
+    steps = 40
+    print(f"[batch] sync_rebuilds over {len(systems)} replicas", flush=True)
+    with sg.Batch([s.params for s in systems], precision="mixed", rebuild_every=0, metal_lanes=4, xcd_map=0,
+                  sync_rebuilds=1) as bt:
+        for k, s in enumerate(systems):
+            bt.upload(k, s.r, s.v, s.a)
+            bt.set_run_state(k, 0, 1, 0.0)
+        recs = bt.run(steps)
+        second = bt.run(steps)  # the counter is back at 0 after every launch
+    for rr in (recs, second):
+        rebuilt = np.array([[x["rebuilt"] for x in r] for r in rr])
+        own = np.array([[x["nebrNow"] for x in r] for r in rr])
+        assert (rebuilt == rebuilt[0]).all()
+        assert np.array_equal(rebuilt[0, 1:], own[:, :-1].any(0))
+        assert all(x["errors"] == 0 for r in rr for x in r)
+    own = np.array([[x["nebrNow"] for x in r] for r in recs])
+    # steps that only the hot replicas (indices 256..299) asked for
+    assert (own[n_cold:, :-1].any(0) & ~own[:n_cold, :-1].any(0)).any()
