@@ -344,7 +344,6 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
     BTRY(balloc(b, &b->state, (size_t)nrep, true));
     BTRY(balloc(b, &b->records, (size_t)nrep * cfg.records_capacity, true));
     BTRY(balloc(b, &b->hist, (size_t)nrep * s0.sizeHistRdf, true));
-    BTRY(balloc(b, &b->syncCount, 1, true));
     BTRY(hipMemcpyAsync(b->dReps, b->reps.data(), sizeof(BatchRep) * nrep, hipMemcpyHostToDevice, b->stream) == hipSuccess
              ? SGK_OK : bfail(b, SGK_E_HIP, "copy of the replica table"));
     BTRY(hipMemcpyAsync(b->atomTab, atomTab.data(), sizeof(int2) * atomTab.size(), hipMemcpyHostToDevice, b->stream) == hipSuccess

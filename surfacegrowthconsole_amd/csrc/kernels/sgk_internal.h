@@ -124,7 +124,6 @@ struct sgk_batch {
     sgk::DevState *state = nullptr;
     sgk_record *records = nullptr;
     unsigned int *hist = nullptr;
-    int *syncCount = nullptr;            // replicas finalized in the current step (sync_rebuilds); 0 between launches
     sgk::DevState *hostState = nullptr;  // pinned mirror, nrep entries
     long long bytes = 0;
     int hostStep = 0;

@@ -17,12 +17,12 @@ namespace sgk {
 // when the device flag asks for a rebuild, with the cell binning of the new position
 // (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
 // kicked: the previous force pass already applied this step's half-kick (integrate 2), so only the
-// drift is left.
+// drift is left.  rebuild: whether this step bins (< 0: the replica's own flag, st->nebrNow).
 template <bool M>
 __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos,
                                          typename Prec<M>::R *__restrict__ vel,
                                          const typename Prec<M>::R *__restrict__ acc, const CellBufs &bufs,
-                                         DevState *st, int doBin, int kicked) {
+                                         DevState *st, int doBin, int kicked, int rebuild = -1) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
     const int n = blk * blockDim.x + threadIdx.x;
@@ -54,7 +54,8 @@ __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename P
     if (r.z >= R(0.5f) * Lz) r.z -= Lz;
     else if (r.z < -R(0.5f) * Lz) r.z += Lz;
     pos[n] = r;
-    if (doBin && st->nebrNow) bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
+    if (doBin && (rebuild < 0 ? st->nebrNow : rebuild))
+        bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
 }
 
 template <bool M>
@@ -66,14 +67,32 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
 }
 
 // Batched: one workgroup of one replica per table entry.
+// syncN > 0 (synchronised rebuilds over syncN replicas): the step rebuilds every replica's lists if
+// any replica's rule asked for it.  Each workgroup ORs the flags the replicas' finalize left, and the
+// first workgroup of each replica stores the OR for the step's later kernels; a workgroup that reads
+// a flag while it is being stored sees the replica's own decision or the OR, so it gets the same OR.
 template <bool M>
-__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked) {
+__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     const int2 e = a.atomTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
+    DevState *st = a.state + e.x;
+    int rebuild = -1;
+    if (syncN > 0) {
+        __shared__ int any;
+        if (threadIdx.x < 64) {
+            int f = 0;
+            for (int r = threadIdx.x; r < syncN; r += 64) f |= a.state[r].nebrNow;
+            f = __any(f) ? 1 : 0;
+            if (threadIdx.x == 0) any = f;
+        }
+        __syncthreads();
+        rebuild = any;
+        if (e.y == 0 && threadIdx.x == 0) st->nebrNow = rebuild;
+    }
     lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
-                rep_cells(a, B), a.state + e.x, 1, kicked);
+                rep_cells(a, B), st, 1, kicked, rebuild);
 }
 
 // InsertAtomsK (SurfaceGrowth.cu:488-514) with the schedule of .cu:2148-2168, for the step about to
@@ -184,36 +203,20 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(DevParams p,
 
 // Batched: both stages in one workgroup per replica - the waves reduce the chunks (the calls of
 // k_reduce1, in the same order), then the workgroup finalizes - so a replica's reduction order is
-// that of the replica run alone.
-// syncCount != nullptr (synchronised rebuilds): the workgroup that finishes last ORs the replicas'
-// rebuild flags into every replica and puts the count back to 0 for the next launch.  The records
-// keep each replica's own decision.
+// that of the replica run alone.  Up to kFinalizeLdsChunks chunk totals stay in LDS.
+constexpr int kFinalizeLdsChunks = 16;
 template <bool M>
-__global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a, int nrep, int *syncCount) {
+__global__ __launch_bounds__(kFinalizeThreads) void kb_finalize(BatchArgs a) {
     using A = typename Prec<M>::A;
+    __shared__ Partial<A> tot[kFinalizeLdsChunks];
     const BatchRep &B = a.reps[blockIdx.x];
     const Partial<A> *part = (const Partial<A> *)a.partials + B.partOff;
-    Partial<A> *part2 = (Partial<A> *)a.partials + B.part2Off;
+    Partial<A> *part2 = B.nChunks <= kFinalizeLdsChunks ? tot : (Partial<A> *)a.partials + B.part2Off;
     for (int j = threadIdx.x >> 6; j < B.nChunks; j += kFinalizeThreads / 64)
         reduce_chunk(part + j * kPartChunk, min(kPartChunk, B.nbForce - j * kPartChunk), part2 + j);
     __syncthreads();
     finalize_body<A>(B.p, part2, B.nChunks, a.state + blockIdx.x, a.records + (size_t)blockIdx.x * a.recCap,
                      a.recCap, 0);
-    if (!syncCount) return;
-    __shared__ int last;
-    __threadfence();  // this replica's state reaches device scope before the count
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(syncCount, 1) == nrep - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    int any = 0;
-    for (int r = threadIdx.x; r < nrep; r += blockDim.x)
-        any |= __hip_atomic_load(&a.state[r].nebrNow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    any = __syncthreads_or(any);
-    if (any)
-        for (int r = threadIdx.x; r < nrep; r += blockDim.x) a.state[r].nebrNow = 1;
-    if (threadIdx.x == 0) *syncCount = 0;
 }
 
 // ApplyBerendsenThermostat (SurfaceGrowth.cu:455-485), kinetic energy of all atoms.
@@ -513,20 +516,20 @@ void launch_diffuse_reset(sgk_ctx *c) {
 
 // ---- batched launchers -----------------------------------------------------------------------
 void launch_batch_lf1(sgk_batch *b, bool kicked) {
+    const int syncN = b->cfg.sync_rebuilds && b->nrep > 1 ? b->nrep : 0;
     if (b->mixed)
-        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked);
+        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN);
     else
-        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked);
+        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN);
 }
 
 // sync_rebuilds: when any replica rebuilds at the next step, all do.  One workgroup, after the
 // replicas' finalize of the step.
 void launch_batch_finalize(sgk_batch *b) {
-    int *sync = b->cfg.sync_rebuilds && b->nrep > 1 ? b->syncCount : nullptr;
     if (b->mixed)
-        hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args(), b->nrep, sync);
+        hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
     else
-        hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args(), b->nrep, sync);
+        hipLaunchKernelGGL(kb_finalize<false>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
 }
 
 void launch_batch_thermostat(sgk_batch *b) {
