@@ -841,10 +841,16 @@ __global__ __launch_bounds__(256) void k_nlc(DevParams p, int nbM, const typenam
                                              int *__restrict__ fullNBL) {
     if (!force && !st->nebrNow) return;
     const int b = fromNext ? (st->cellbuf ^ 1) : st->cellbuf;
-    if ((int)blockIdx.x < nbM)
-        nlm_body<M, kNlMetalLanes, KEEP>(p, blockIdx.x, pos, bufs, b, st, order, nlM, nnM, fullNN, fullNBL);
+    // xcd_map bit 0: each XCD walks a contiguous 1/8 of each range, so the cells a workgroup shares
+    // with the next ones in the list-build order stay in one L2 (the lists do not depend on it)
+    const bool xm = p.xcdMap & 1;
+    const int bb = blockIdx.x;
+    if (bb < nbM)
+        nlm_body<M, kNlMetalLanes, KEEP>(p, xm ? xcd_block(bb, nbM) : bb, pos, bufs, b, st, order, nlM, nnM, fullNN,
+                                         fullNBL);
     else
-        nlc_body<M, KEEP>(p, blockIdx.x - nbM, pos, bufs, b, st, order, lj, nlj, cl, ncl, fullNN, fullNBL);
+        nlc_body<M, KEEP>(p, xm ? xcd_block(bb - nbM, gridDim.x - nbM) : bb - nbM, pos, bufs, b, st, order, lj, nlj,
+                          cl, ncl, fullNN, fullNBL);
 }
 
 // ---- list-build order (nl_scheme 3) ----------------------------------------------------------------

@@ -42,9 +42,10 @@ def force_error(acc, ref):
     return np.abs(acc - ref).max() / rms
 
 
-def check_lists(p, prec, scheme, r, ordered=False):
+def check_lists(p, prec, scheme, r, ordered=False, xcd_map=None):
     o = oracle.Oracle(p, "f32")
-    with sg.Engine(p, precision=prec, keep_full_nl=True, nl_scheme=scheme) as eng:
+    opts = {} if xcd_map is None else {"xcd_map": xcd_map}
+    with sg.Engine(p, precision=prec, keep_full_nl=True, nl_scheme=scheme, **opts) as eng:
         eng.upload(r)
         if ordered:
             eng.build_nl_order()
@@ -95,6 +96,16 @@ def test_neighbour_list_bit_exact_thermalised(gpu, name, scheme):
     r, _, _ = thermalised(name)
     print(f"[parity] NL thermalised {name} scheme {scheme}", flush=True)
     check_lists(s.params, "mixed", scheme, r, ordered=scheme == 3)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_cell_ordered_lists_with_the_xcd_remap(gpu, prec):
+    """xcd_map bit 0: the cell-ordered walk's workgroups, remapped so that each XCD takes a contiguous
+    eighth of the metal and of the carbon ranges, build the same lists."""
+    s = system("medium")
+    r, _, _ = thermalised("medium")
+    print(f"[parity] NL cell-ordered with XCD remap {prec}", flush=True)
+    check_lists(s.params, prec, 3, r, ordered=True, xcd_map=3)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
