@@ -590,6 +590,8 @@ def main():
                      "kernel": "k_force (fused metal+carbon force pass)", "us_per_launch": force_us,
                      "alg_bytes_per_atom_step": alg_per_atom,
                      "hbm_traffic_frac": hbm_frac, "valu_issue_frac": valu_frac,
+                     "counters_from": [f"profiles/k_force_{args.config}_traffic.md",
+                                       f"profiles/k_force_{args.config}_valu.md"] if traffic else None,
                      "step_frac": alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "samples": samples,
         "kernels": kernels,
