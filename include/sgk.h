@@ -52,7 +52,9 @@ typedef struct sgk_config {
     int close_capacity;   /* close neighbours kept per carbon atom; 0 -> 16 */
     float close_margin;   /* close radius = rCutC + 2*(0.5*rNebrShell + margin); 0 -> 0.2 */
     int metal_lanes;      /* lanes per metal atom in the rho/force kernels (1..16); 0 -> auto */
-    int block_size;       /* threads per block for per-atom kernels; 0 -> 256 */
+    int block_size;       /* threads per workgroup of the per-atom kernels: 0 or 256 (the kernels, their
+                             launch bounds and reductions are written for 4-wave workgroups; any other
+                             value is rejected with SGK_E_UNSUPPORTED) */
     int keep_full_nl;     /* also store the neighbour list in the reference layout (tests) */
     int rebuild_every;    /* 0: adaptive rule dispHi > rNebrShell/2 only; k>0: also every k steps */
     int records_capacity; /* step records buffered on the device; 0 -> 1024 */

@@ -236,6 +236,8 @@ int sgk::prepare_ctx(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx *c, s
     if (cfg.cell_capacity > 64 || cfg.cell_capacity % 8)
         return bad(SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64 (one wave per cell)");
     if (cfg.close_capacity % 8) return bad(SGK_E_UNSUPPORTED, "close capacity must be a multiple of 8");
+    if (cfg.block_size != 256)
+        return bad(SGK_E_UNSUPPORTED, "block size must be 256 (the kernels are written for 4-wave workgroups)");
     if (sp->cells.x <= 0 || sp->cells.y <= 0 || sp->cells.z <= 0) return bad(SGK_E_ARG, "empty cell grid");
     if ((long long)sp->cells.x * sp->cells.y * sp->cells.z * cfg.cell_capacity >= (1ll << 31))
         return bad(SGK_E_UNSUPPORTED, "cell grid too large: cells x cell capacity must stay below 2^31");

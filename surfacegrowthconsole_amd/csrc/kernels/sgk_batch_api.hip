@@ -208,6 +208,9 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
     if (cfg.close_capacity <= 0) cfg.close_capacity = def.close_capacity;
     if (cfg.close_margin <= 0.f) cfg.close_margin = def.close_margin;
     if (cfg.records_capacity <= 0) cfg.records_capacity = def.records_capacity;
+    if (cfg.block_size <= 0) cfg.block_size = def.block_size;
+    if (cfg.block_size != 256)
+        return bfail(nullptr, SGK_E_UNSUPPORTED, "block size must be 256 (the kernels are written for 4-wave workgroups)");
     if (cfg.nl_capacity > 255) return bfail(nullptr, SGK_E_UNSUPPORTED, "neighbour capacity above 255");
     if (cfg.cell_capacity > 64 || cfg.cell_capacity % 8)
         return bfail(nullptr, SGK_E_UNSUPPORTED, "cell capacity must be a multiple of 8 up to 64");

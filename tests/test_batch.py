@@ -34,6 +34,31 @@ def test_batch_rejects_replicas_with_different_potentials():
     assert b"LJ" in lib.sgk_batch_last_error(None)
 
 
+def test_batch_rejects_workgroups_other_than_256_threads():
+    """The kernels are written for 4-wave workgroups: block_size other than 0 or 256 is an error,
+    raised before any device work."""
+    a = sg.config_system("small", "config")
+    arr = (N.SimParams * 1)(a.params)
+    lib = N.lib()
+    for bs in (128, 512):
+        cfg = N.SgkConfig()
+        lib.sgk_default_config(C.byref(cfg))
+        cfg.block_size = bs
+        h = C.c_void_p()
+        rc = lib.sgk_batch_create(1, C.cast(arr, C.c_void_p), C.byref(cfg), C.byref(h))
+        assert rc != 0 and not h.value, bs
+        assert b"block size" in lib.sgk_batch_last_error(None)
+
+
+@pytest.mark.gpu
+def test_engine_rejects_workgroups_other_than_256_threads(gpu):
+    s = sg.config_system("small", "config")
+    with pytest.raises(RuntimeError, match="block size must be 256"):
+        sg.Engine(s.params, block_size=128)
+    with sg.Engine(s.params, block_size=0) as eng:  # 0 takes the default, 256
+        assert eng.cfg.block_size == 0
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
 @pytest.mark.parametrize("nl_scheme", [3, 2, 0])
