@@ -523,8 +523,8 @@ void launch_batch_lf1(sgk_batch *b, bool kicked) {
         hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN);
 }
 
-// sync_rebuilds: when any replica rebuilds at the next step, all do.  One workgroup, after the
-// replicas' finalize of the step.
+// One workgroup per replica closes the step (the rebuild flags of synchronised rebuilds are ORed at
+// the start of the next step, in kb_lf1).
 void launch_batch_finalize(sgk_batch *b) {
     if (b->mixed)
         hipLaunchKernelGGL(kb_finalize<true>, dim3(b->nrep), dim3(kFinalizeThreads), 0, b->stream, b->args());
