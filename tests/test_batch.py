@@ -87,6 +87,43 @@ def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_batch_with_a_metal_free_and_a_one_atom_replica(gpu, prec):
+    """A replica without metal (no metal workgroups in any pass) and one with a single metal atom,
+    beside an ordinary one, with synchronised rebuilds: each equals the replica run alone with the
+    same rebuild steps, bit for bit."""
+    systems = [sg.config_system("small", "config", ux=4, uy=4, n_metal=0, temperature=600.0, seed=3),
+               sg.config_system("small", "config", ux=4, uy=5, n_metal=1, temperature=900.0, seed=4),
+               sg.config_system("small", "config", ux=5, uy=5, n_metal=147, temperature=300.0, seed=5)]
+    assert [s.params.nMolMe for s in systems] == [0, 1, 147]
+    steps = 40
+    print(f"[batch] {prec}: replicas with 0, 1 and 147 metal atoms", flush=True)
+    with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=0, metal_lanes=4, xcd_map=0,
+                  sync_rebuilds=1) as bt:
+        for k, s in enumerate(systems):
+            bt.upload(k, s.r, s.v, s.a)
+            bt.set_run_state(k, 0, 1, 0.0)
+        recs = bt.run(steps)
+        out = [bt.download(k) for k in range(len(systems))]
+    rebuilt = [x["rebuilt"] for x in recs[0]]
+    for k, s in enumerate(systems):
+        with sg.Engine(s.params, precision=prec, rebuild_every=0, metal_lanes=4, xcd_map=0,
+                       records_capacity=1) as eng:
+            eng.upload(s.r, s.v, s.a)
+            eng.set_run_state(0, 1, 0.0)
+            ref = []
+            for step in range(steps):
+                if step > 0 and rebuilt[step]:
+                    eng.set_run_state(step, 1, ref[-1]["dispHi"])
+                ref += eng.run(1)
+            r, v, _ = eng.download()
+        assert [x["rebuilt"] for x in ref] == rebuilt, k
+        assert np.array_equal(out[k][0][:, :3], r[:, :3]), k
+        assert np.array_equal(out[k][1], v), k
+        assert [x["totEnergy"] for x in recs[k]] == [x["totEnergy"] for x in ref], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_batch_with_lanes_per_replica_equals_replicas_run_alone(gpu, prec):
     """Replicas with different lanes per metal atom in one batch (one density/force launch per lane
     count): each equals the replica run alone with its own lanes, bit for bit."""
