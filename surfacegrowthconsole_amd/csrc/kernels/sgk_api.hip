@@ -254,8 +254,7 @@ int sgk::prepare_ctx(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx *c, s
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) G = p.nMolMe >= 65536 ? 4 : p.nMolMe >= 2048 ? 8 : 16;
     c->metalLanes = G;
     c->block = 256;
-    // metal ranges padded to a multiple of 8 workgroups so the carbon range starts on XCD 0
-    c->nbM = (int)((((long long)p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+    c->nbM = metal_blocks(p.nMolMe, G, p.xcdMap);
     c->nbC = (p.nC + 255) / 256;
     c->nbRho = c->nbM;
     c->nbProps = (p.nMol + 255) / 256;
@@ -1069,9 +1068,10 @@ int sgk_set_deposited(sgk_ctx *c, int nMolDeposited) {
     return SGK_OK;
 }
 
-static void set_metal_lanes(sgk_ctx *c, int G) {
+static void set_shape(sgk_ctx *c, int G, int xcdMap) {
     c->metalLanes = G;
-    c->nbM = (int)((((long long)c->dp.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+    c->dp.xcdMap = xcdMap;
+    c->nbM = metal_blocks(c->dp.nMolMe, G, xcdMap);
     c->nbRho = c->nbM;
 }
 
@@ -1102,8 +1102,7 @@ int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, 
     for (int G : lanes) {
         if (c->dp.nMolMe == 0 && G != lanes[0]) break;  // no metal pass to shape
         for (int m : maps) {
-            set_metal_lanes(c, G);
-            c->dp.xcdMap = m;
+            set_shape(c, G, m);
             sgk::launch_rho(c);  // warm-up
             sgk::launch_force(c, false, false);
             SGK_HIP(c, hipEventRecord(e0, c->stream));
@@ -1128,8 +1127,7 @@ int sgk_autotune(sgk_ctx *c, int reps, sgk_tune_entry *entries, int maxEntries, 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_metal_lanes(c, bestG);
-    c->dp.xcdMap = bestMap;
+    set_shape(c, bestG, bestMap);
     c->cfg.metal_lanes = bestG;
     c->cfg.xcd_map = bestMap;
     // leave acceleration, energy and friction as the chosen shape computes them

@@ -125,7 +125,7 @@ std::vector<int2> force_layout(sgk_batch *b) {
     for (int r = 0; r < b->nrep; ++r) {
         BatchRep &R = b->reps[r];
         const int G = b->repLanes[r];
-        R.nbM = (int)((((long long)R.p.nMolMe * G + 255) / 256 + 7) / 8 * 8);
+        R.nbM = metal_blocks(R.p.nMolMe, G, 0);  // the batch remaps nothing
         R.nbForce = R.nbM + (R.p.nC + 255) / 256;
         R.partOff = partOff;
         R.nChunks = (R.nbForce + 63) / 64;  // sgk::kPartChunk partials per chunk

@@ -90,6 +90,14 @@ struct sgk_ctx {
 };
 
 // A batch of replicas advanced together (sgk_batch_*): shared arrays, one range per replica.
+// Force-pass workgroups of nMe metal atoms at G lanes each.  With the density or force pass
+// remapped to XCDs (xcd_map bit 1 or 2) the count is padded to a multiple of 8, so that the carbon
+// range starts on XCD 0; without, the padding workgroups would only idle.
+inline int metal_blocks(long long nMe, int G, int xcdMap) {
+    const long long nb = (nMe * G + 255) / 256;
+    return (int)((xcdMap & 6) ? (nb + 7) / 8 * 8 : nb);
+}
+
 struct sgk_batch {
     int nrep = 0;
     std::vector<SimParams> sp;
