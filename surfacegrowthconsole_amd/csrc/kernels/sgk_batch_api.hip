@@ -275,9 +275,9 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         const int nbC = (R.p.nC + 255) / 256;
         for (int k = 0; k < nbAtom; ++k) atomTab.push_back(make_int2(r, k));
         for (int k = 0; k < nbMetal; ++k) metalTab.push_back(make_int2(r, k));
-        // cell-ordered list build: kNlMetalLanes (4) lanes per metal atom, then one lane per carbon;
+        // cell-ordered list build: kNlMetalLanes lanes per metal atom, then one lane per carbon;
         // every replica's metal workgroups (the long walks) are dispatched before any carbon one
-        R.nbNlM = (int)(((long long)s.nMolMe * 4 + 255) / 256);
+        R.nbNlM = (int)(((long long)s.nMolMe * sgk::kNlMetalLanes + 255) / 256);
         for (int k = 0; k < R.nbNlM; ++k) nlTab.push_back(make_int2(r, k));
         for (int k = 0; k < nbC; ++k) nlTabC.push_back(make_int2(r, R.nbNlM + k));
         R.ordOff = (int)ordOff;

@@ -636,7 +636,6 @@ __global__ __launch_bounds__(256) void k_nlt(DevParams p, const typename Prec<M>
 //   cell it took two cells padded to 8.  Each candidate is one 16-byte load (position and atom
 //   index); there is no self test outside the atom's own column unless a short grid axis wraps a
 //   neighbour offset onto it.
-constexpr int kNlMetalLanes = 4;  // each takes 2 of every chunk of 8 candidates
 
 // Lists of metal atom i, G lanes per atom (l = lane within the group); called by whole groups.  Per
 // group step the lanes test candidates base + l and base + G + l of the column (base a multiple of 8).
@@ -645,7 +644,7 @@ __device__ __forceinline__ void nlm_body(const DevParams &p, int blk, const type
                                          const CellBufs &bufs, int b, DevState *st, const int *__restrict__ order,
                                          int *__restrict__ nlM, int *__restrict__ nnM, int *__restrict__ fullNN,
                                          int *__restrict__ fullNBL) {
-    static_assert(G == 4, "two candidates per lane cover a chunk of 8");
+    static_assert(G == 4 || G == 8 || G == 16, "two candidates per lane, 2G a step");
     const int gt = blk * blockDim.x + threadIdx.x;
     const int g = gt / G, l = gt % G;
     const bool active = g < p.nMolMe;
