@@ -21,7 +21,7 @@ counter summaries); `step_frac` is the same algorithmic bytes over the whole ste
 
 cpu_baseline: the FP32 CPU restatement of the same step (oracle/), built for speed (-O3
 -march=x86-64-v3, OpenMP over the per-atom loops), on one core and on the host's cores, each on a
-bounded number of steps of the same workload.  The FP64 oracle is the checker for the force and
+bounded number of steps of the same workload.  The FP64 oracle gives the force and
 energy error of both GPU forms, on the initial state and on the state the timed mixed run ends in.
 """
 import argparse

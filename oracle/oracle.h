@@ -1,7 +1,7 @@
 /*
  * oracle.h - CPU restatement of the SurfaceGrowth per-step MD algorithm.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine and the CPU baseline
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness reference for the HIP engine and the CPU baseline
  * leg of bench.py; nothing in the product links, loads or calls it.
  *
  * Every function restates a reference routine, cited by file:line (paths relative to the

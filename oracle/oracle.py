@@ -1,6 +1,6 @@
 """ctypes loader of the CPU oracle (oracle/_build/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the checker and by
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to check its result and by
 bench.py's cpu_baseline leg.  The product never imports this module.
 """
 import ctypes as C
@@ -74,7 +74,7 @@ def timing_lib():
 
 
 def threaded_f64_lib():
-    """The FP64 oracle with OpenMP per-atom loops, built with the checker's flags: its *_f64 results
+    """The FP64 oracle with OpenMP per-atom loops, built with liboracle's flags: its *_f64 results
     are bitwise those of lib() (each atom's sums keep their order); for long FP64 runs."""
     global _omp64
     if _omp64 is None:

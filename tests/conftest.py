@@ -27,5 +27,5 @@ def gpu():
 
     n = _native.lib().sgk_device_count()
     if n == 0:
-        pytest.fail("no HIP device visible: gpu tests must run on the MI355X box")
+        pytest.fail("no HIP device visible: gpu tests need an MI355X")
     return n

@@ -5,7 +5,7 @@ test time, edited in a temporary directory with the edits CPP_EDITS and PROTO_ED
 driver's CUDA device queries, its Windows entry point and key wait, and its header include), compiled
 with g++ against include/surfacegrowth_driver.h, and linked against libsurfacegrowth.so.  Nothing of
 the reference is written into the repository and the linked program is never run.  Skipped where
-the reference tree is absent (the GPU box).
+the reference tree is absent.
 """
 import os
 import re
