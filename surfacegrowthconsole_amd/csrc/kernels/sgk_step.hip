@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
 // any replica's rule asked for it.  Each workgroup ORs the flags the replicas' finalize left, and the
 // first workgroup of each replica stores the OR for the step's later kernels; a workgroup that reads
 // a flag while it is being stored sees the replica's own decision or the OR, so it gets the same OR.
+// The flag is read and stored with relaxed atomics, so that overlap is defined behaviour.
 template <bool M>
 __global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN) {
     using P4 = typename Prec<M>::P4;
@@ -83,13 +84,15 @@ __global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN
         __shared__ int any;
         if (threadIdx.x < 64) {
             int f = 0;
-            for (int r = threadIdx.x; r < syncN; r += 64) f |= a.state[r].nebrNow;
+            for (int r = threadIdx.x; r < syncN; r += 64)
+                f |= __hip_atomic_load(&a.state[r].nebrNow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             f = __any(f) ? 1 : 0;
             if (threadIdx.x == 0) any = f;
         }
         __syncthreads();
         rebuild = any;
-        if (e.y == 0 && threadIdx.x == 0) st->nebrNow = rebuild;
+        if (e.y == 0 && threadIdx.x == 0)
+            __hip_atomic_store(&st->nebrNow, rebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
                 rep_cells(a, B), st, 1, kicked, rebuild);
