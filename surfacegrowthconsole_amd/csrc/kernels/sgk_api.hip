@@ -331,6 +331,7 @@ int sgk_create(const SimParams *sp, const sgk_config *cfgIn, sgk_ctx **out) {
     const size_t R = c->mixed ? 8 : 4;
     const size_t nMol = p.nMol, nMe = std::max(p.nMolMe, 1), nC = std::max(p.nC, 1);
     SGK_TRY(dalloc(c, (char **)&c->pos, nMol * (c->mixed ? 32 : 16), true));
+    SGK_TRY(dalloc(c, (char **)&c->posAlt, nMol * (c->mixed ? 32 : 16), true));
     SGK_TRY(dalloc(c, (char **)&c->vel, 3 * nMol * R, true));
     SGK_TRY(dalloc(c, (char **)&c->acc, 3 * nMol * R, true));
     SGK_TRY(dalloc(c, &c->rho, nMe, true));
@@ -419,7 +420,7 @@ int sgk_destroy(sgk_ctx *c) {
     }
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-    void *ptrs[] = {c->pos, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
+    void *ptrs[] = {c->pos, c->posAlt, c->vel, c->acc, c->rho, c->embF, c->embDF, c->fric, c->cellOf[0], c->cellOf[1],
                     c->cnt[0], c->cnt[1], c->slots[0], c->slots[1], c->slotOf, c->cellPos, c->nlM, c->nnM, c->lj, c->nlj, c->cl,
                     c->ncl, c->bondC, c->bondA, c->fullNN, c->fullNBL, c->partials, c->state, c->records, c->hist,
                     c->nlOrder, c->ordCnt, c->ordKey, c->ordSlot, c->ordBlock, c->rdfCnt, c->rdfBlock, c->rdfKey,
@@ -902,22 +903,49 @@ static void enqueue_steps(sgk_ctx *c, const std::string &flags) {
     // The force pass of a step also applies the next step's first half-kick when nothing touches the
     // velocities or accelerations in between (no thermostat, shear or deposition) and the step is
     // not the chunk's last (the host then reads full-step velocities): the next k_lf1 only drifts.
-    bool kicked = false;
-    for (size_t k = 0; k < flags.size(); ++k) {
+    // Such a force pass can also do that drift itself ("fused"): it writes the next positions into
+    // posAlt, since the other atoms of the same pass still read the current ones, and the next step
+    // starts with the binning alone (k_bin, which returns at once unless the lists are rebuilt).
+    // The host swaps pos and posAlt after each fused step, so every launch here takes the current
+    // buffer.  An even number of fused steps per chunk (the first pre-kicked step stays unfused
+    // when the count is odd) returns the positions to the buffer the chunk started from: a
+    // captured graph then starts and ends on the same buffer, and code outside the step loop sees
+    // one position buffer.
+    const size_t n = flags.size();
+    std::vector<char> pre(n, 0);
+    int nPre = 0;
+    for (size_t k = 0; k < n; ++k) {
         const int b = flags[k] - 'a';
-        const bool prekick = c->dp.iRegime != 1 && !(b & 3) && k + 1 < flags.size();
+        pre[k] = c->dp.iRegime != 1 && !(b & 3) && k + 1 < n;
+        nPre += pre[k];
+    }
+    bool unfuseFirst = (nPre & 1) != 0;
+    bool kicked = false, drifted = false;
+    for (size_t k = 0; k < n; ++k) {
+        const int b = flags[k] - 'a';
+        bool fuse = false;
+        if (pre[k] && c->posAlt) {
+            fuse = !unfuseFirst;
+            unfuseFirst = false;
+        }
         sgk::launch_insert(c);  // regime 1 only
-        sgk::launch_lf1(c, true, kicked);
+        if (drifted)
+            sgk::launch_bin(c, false);
+        else
+            sgk::launch_lf1(c, true, kicked);
         sgk::launch_sortcells(c, false);
         sgk::launch_nl(c, false);
         sgk::launch_rho(c);
-        sgk::launch_force(c, true, true, prekick);
-        kicked = prekick;
+        sgk::launch_force(c, true, true, pre[k], fuse ? c->posAlt : nullptr);
+        kicked = pre[k];
+        drifted = fuse;
         if (b & 1) sgk::launch_thermostat(c, 0, 0.0, 0);
         if (b & 2) sgk::launch_shear(c);
+        // the samples below read this step's positions, still in pos
         if (b & 4) sgk::launch_rdf(c);
         if (b & 16) sgk::launch_diffuse(c);
         if (b & 8) sgk::launch_adh_obs(c, c->adj->d.part, c->adj->d.adh, 1);
+        if (fuse) std::swap(c->pos, c->posAlt);
     }
 }
 

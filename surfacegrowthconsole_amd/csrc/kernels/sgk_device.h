@@ -40,6 +40,24 @@ __device__ __forceinline__ R wrap1(R d, R L) {
     return d;
 }
 
+// Leapfrog drift and periodic wrap of one atom (LeapfrogStepK part 1 after its half-kick, and
+// ApplyBoundaryCondK: SurfaceGrowth.cu:397-453): r + dt v if the atom moves, then each coordinate
+// wrapped once into [-L/2, L/2).  k_lf1 and the force pass's fused drift both call it, so the two
+// paths round alike.
+template <class R, class P4>
+__device__ __forceinline__ void drift_wrap(const DevParams &p, P4 &r, bool move, R vx, R vy, R vz) {
+#pragma clang fp contract(off)
+    if (move) {
+        const R dt = (R)p.deltaT;
+        r.x = r.x + dt * vx;
+        r.y = r.y + dt * vy;
+        r.z = r.z + dt * vz;
+    }
+    r.x = wrap1<R>(r.x, (R)p.Lx);
+    r.y = wrap1<R>(r.y, (R)p.Ly);
+    r.z = wrap1<R>(r.z, (R)p.Lz);
+}
+
 // gfx950 dispatches workgroups to its 8 XCDs round robin: workgroup b runs on XCD b % 8, and each
 // XCD has its own L2.  xcd_block maps the hardware index b of a range of n workgroups to a logical
 // index so that every XCD works on one contiguous 1/8 of the range.  Atoms with nearby indices are

@@ -202,9 +202,10 @@ __global__ __launch_bounds__(256) void kb_rho(BatchArgs a) {
 // and per-atom sums in T.  Returns the atom's contribution to the block reductions.
 template <bool M, class T, int G>
 __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
-    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
-    typename Prec<M>::R *__restrict__ acc, const int *__restrict__ nlM, const int *__restrict__ nnM, const float *__restrict__ embF,
-    const float *__restrict__ embDF, float *__restrict__ fric, int integrate) {
+    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::P4 *__restrict__ posOut,
+    typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc, const int *__restrict__ nlM,
+    const int *__restrict__ nnM, const float *__restrict__ embF, const float *__restrict__ embDF,
+    float *__restrict__ fric, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
     using P4 = typename Prec<M>::P4;
@@ -283,9 +284,9 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
             acc[2 * p.nMol + i] = az;
         }
         const T e = T(0.5f) * u + (T)embF[i];
-        pos[i].w = (R)e;
         fric[i] = (float)frx;
         R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
+        R nx = vx, ny = vy, nz = vz;
         if (integrate) {
 #pragma clang fp contract(off)
             const R hdt = (R)(0.5f * p.deltaT);
@@ -293,9 +294,20 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> metal_body(
             vy = vy + hdt * ay;
             vz = vz + hdt * az;
             // integrate 2: also the next step's first half-kick, the same operations k_lf1 would do
-            vel[i] = integrate == 2 ? vx + hdt * ax : vx;
-            vel[p.nMol + i] = integrate == 2 ? vy + hdt * ay : vy;
-            vel[2 * p.nMol + i] = integrate == 2 ? vz + hdt * az : vz;
+            nx = integrate == 2 ? vx + hdt * ax : vx;
+            ny = integrate == 2 ? vy + hdt * ay : vy;
+            nz = integrate == 2 ? vz + hdt * az : vz;
+            vel[i] = nx;
+            vel[p.nMol + i] = ny;
+            vel[2 * p.nMol + i] = nz;
+        }
+        if (posOut) {  // integrate 2 with the next step's drift: the position it starts from
+            P4 rn = Ai;
+            rn.w = (R)e;
+            drift_wrap<R>(p, rn, true, nx, ny, nz);
+            posOut[i] = rn;
+        } else {
+            pos[i].w = (R)e;
         }
         const A m = (A)p.massMe;
         q.u = (A)e;
@@ -393,9 +405,9 @@ __device__ __forceinline__ void neighbour_angle(const DevParams &p, const V3<R> 
 // ComputeForcesK carbon branch (SurfaceGrowth.cu:899-1062, 1083-1088), one thread per carbon.
 template <bool M, class T>
 __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
-    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::R *__restrict__ vel,
-    typename Prec<M>::R *__restrict__ acc, const unsigned *__restrict__ lj, const int *__restrict__ nlj,
-    const uint4 *__restrict__ bondC, const uint4 *__restrict__ bondA, int integrate) {
+    const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos, typename Prec<M>::P4 *__restrict__ posOut,
+    typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc, const unsigned *__restrict__ lj,
+    const int *__restrict__ nlj, const uint4 *__restrict__ bondC, const uint4 *__restrict__ bondA, int integrate) {
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
     using P4 = typename Prec<M>::P4;
@@ -527,8 +539,8 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         acc[2 * p.nMol + i] = az;
     }
     const float e = (float)(T(0.5f) * gu);
-    pos[i].w = (R)e;
     R vx = vel[i], vy = vel[p.nMol + i], vz = vel[2 * p.nMol + i];
+    R nx = vx, ny = vy, nz = vz;
     const bool move = Ai.x != (R)p.xLeft && Ai.x < (R)p.xRight && Ai.y != (R)p.yTop && Ai.y != (R)p.yBottom;
     if (integrate && move) {
 #pragma clang fp contract(off)
@@ -536,9 +548,20 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
         vx = vx + hdt * ax;
         vy = vy + hdt * ay;
         vz = vz + hdt * az;
-        vel[i] = integrate == 2 ? vx + hdt * ax : vx;  // 2: with the next step's first half-kick
-        vel[p.nMol + i] = integrate == 2 ? vy + hdt * ay : vy;
-        vel[2 * p.nMol + i] = integrate == 2 ? vz + hdt * az : vz;
+        nx = integrate == 2 ? vx + hdt * ax : vx;  // 2: with the next step's first half-kick
+        ny = integrate == 2 ? vy + hdt * ay : vy;
+        nz = integrate == 2 ? vz + hdt * az : vz;
+        vel[i] = nx;
+        vel[p.nMol + i] = ny;
+        vel[2 * p.nMol + i] = nz;
+    }
+    if (posOut) {  // integrate 2 with the next step's drift (k_lf1's move test: Ai is its position)
+        P4 rn = Ai;
+        rn.w = (R)e;
+        drift_wrap<R>(p, rn, move, nx, ny, nz);
+        posOut[i] = rn;
+    } else {
+        pos[i].w = (R)e;
     }
     q.u = (A)e;
     q.vir = (A)(T(0.5f) * gw);
@@ -556,6 +579,7 @@ __device__ __forceinline__ Partial<typename Prec<M>::A> carbon_body(
 template <bool M, class T, int G>
 __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, int nb, bool xm,
                                             typename Prec<M>::P4 *__restrict__ pos,
+                                            typename Prec<M>::P4 *__restrict__ posOut,
                                             typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
                                             const int *__restrict__ nlM, const int *__restrict__ nnM,
                                             const float *__restrict__ embF, const float *__restrict__ embDF,
@@ -567,13 +591,13 @@ __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, 
     // Each branch reduces and stores on its own: a Partial merged across the two branches keeps
     // both bodies' registers live at once (154 VGPRs in the FP64 form against 120 apart).
     if (b < nbM) {
-        const Partial<A> q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, vel, acc, nlM, nnM, embF, embDF,
-                                                 fric, integrate);
+        const Partial<A> q = metal_body<M, T, G>(p, xm ? xcd_block(b, nbM) : b, pos, posOut, vel, acc, nlM, nnM, embF,
+                                                 embDF, fric, integrate);
         const Partial<A> r = block_reduce_partial<true>(q);
         if (threadIdx.x == 0) part[b] = r;
     } else {
-        const Partial<A> q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, vel, acc, lj, nlj,
-                                               bondC, bondA, integrate);
+        const Partial<A> q = carbon_body<M, T>(p, xm ? xcd_block(b - nbM, nb - nbM) : b - nbM, pos, posOut, vel, acc,
+                                               lj, nlj, bondC, bondA, integrate);
         const Partial<A> r = block_reduce_partial<false>(q);
         if (threadIdx.x == 0) part[b] = r;
     }
@@ -581,6 +605,7 @@ __device__ __forceinline__ void force_block(const DevParams &p, int b, int nbM, 
 
 template <bool M, class T, int G>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_force(DevParams p, int nbM, typename Prec<M>::P4 *__restrict__ pos,
+                                               typename Prec<M>::P4 *__restrict__ posOut,
                                                typename Prec<M>::R *__restrict__ vel, typename Prec<M>::R *__restrict__ acc,
                                                const int *__restrict__ nlM, const int *__restrict__ nnM,
                                                const float *__restrict__ embF, const float *__restrict__ embDF,
@@ -589,8 +614,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                const uint4 *__restrict__ bondA, Partial<typename Prec<M>::A> *part,
                                                int integrate) {
     // nbM is a multiple of 8: both ranges keep the XCD of each block
-    force_block<M, T, G>(p, blockIdx.x, nbM, gridDim.x, p.xcdMap & 4, pos, vel, acc, nlM, nnM, embF, embDF, fric,
-                         lj, nlj, bondC, bondA, part, integrate);
+    force_block<M, T, G>(p, blockIdx.x, nbM, gridDim.x, p.xcdMap & 4, pos, posOut, vel, acc, nlM, nnM, embF, embDF,
+                         fric, lj, nlj, bondC, bondA, part, integrate);
 }
 
 // Batched: each replica's workgroups write into its own range of partials (kb_finalize reduces).
@@ -601,7 +626,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int2 e = a.forceTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
     const DevParams &p = B.p;
-    force_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (typename Prec<M>::P4 *)a.pos + B.atomOff,
+    force_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (typename Prec<M>::P4 *)a.pos + B.atomOff, nullptr,
                          (R *)a.vel + 3 * B.atomOff, (R *)a.acc + 3 * B.atomOff, a.nlM + B.metalOff * p.nlCap,
                          a.nnM + B.metalOff, a.embF + B.metalOff, a.embDF + B.metalOff, a.fric + B.metalOff,
                          a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.bondC + B.carbonOff,
@@ -630,7 +655,7 @@ static void rho_dispatch(sgk_ctx *c) {
 }
 
 template <bool M, class T>
-static void force_dispatch(sgk_ctx *c, int integrate) {
+static void force_dispatch(sgk_ctx *c, int integrate, void *posOut) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     using A = typename Prec<M>::A;
@@ -640,7 +665,8 @@ static void force_dispatch(sgk_ctx *c, int integrate) {
     R *vel = (R *)c->vel;
     Partial<A> *part = (Partial<A> *)c->partials;
 #define SGK_F(GG)                                                                                                \
-    hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, vel, (R *)c->acc, \
+    hipLaunchKernelGGL((k_force<M, T, GG>), dim3(g), dim3(256), 0, c->stream, c->dp, c->nbM, pos, (P4 *)posOut,  \
+                       vel, (R *)c->acc,                                                                        \
                        c->nlM, c->nnM, c->embF, c->embDF, c->fric, c->lj, c->nlj, c->bondC, c->bondA, part,     \
                        integrate)
     switch (G) {
@@ -709,16 +735,17 @@ void launch_batch_rho(sgk_batch *b) { batch_launch(b, false, 0); }
 void launch_batch_force(sgk_batch *b, bool prekick) { batch_launch(b, true, prekick ? 2 : 1); }
 void launch_batch_force_eval(sgk_batch *b) { batch_launch(b, true, 0); }
 
-void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick) {
+void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick, void *posOut) {
     if (c->nbM + c->nbC == 0) return;
     profile_begin(c);
     const int mode = integrate ? (prekick ? 2 : 1) : 0;
+    if (mode != 2) posOut = nullptr;  // the drift follows the next step's half-kick only
     if (!c->mixed)
-        force_dispatch<false, float>(c, mode);
+        force_dispatch<false, float>(c, mode, posOut);
     else if (c->cfg.pair_fp64)
-        force_dispatch<true, double>(c, mode);
+        force_dispatch<true, double>(c, mode, posOut);
     else
-        force_dispatch<true, float>(c, mode);
+        force_dispatch<true, float>(c, mode, posOut);
     profile_end(c, "force");
     if (finalize) launch_finalize(c, 0);
 }

@@ -41,6 +41,7 @@ struct sgk_ctx {
 
     // state arrays (positions AoS x,y,z,E; velocity and acceleration SoA, storage precision R)
     void *pos = nullptr;   // float4 or double4 [nMol]
+    void *posAlt = nullptr;  // the same: where a force pass with the fused drift writes the next positions
     void *vel = nullptr;   // R [3*nMol]
     void *acc = nullptr;   // R [3*nMol] (FP64 in the mixed form)
     float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;  // [nMolMe]
@@ -216,7 +217,8 @@ void launch_batch_force(sgk_batch *b, bool prekick = false);
 void launch_batch_force_eval(sgk_batch *b);  // forces, energies and partials only: velocities untouched
 // prekick: the force pass also applies the next step's first half-kick (the caller guarantees that
 // nothing touches velocities or accelerations before that step's launch_lf1(kicked = true)).
-void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick = false);
+// posOut (with prekick): also drift the positions of the next step into posOut (sgk_force.hip).
+void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick = false, void *posOut = nullptr);
 // sgk_step.hip
 void launch_lf1(sgk_ctx *c, bool bin, bool kicked = false);
 void launch_insert(sgk_ctx *c);

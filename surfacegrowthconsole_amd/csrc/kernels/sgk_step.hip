@@ -31,9 +31,12 @@ __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename P
     bool move = true;
     if (n >= p.nMolMe)
         move = r.x != (R)p.xLeft && r.x < (R)p.xRight && r.y != (R)p.yTop && r.y != (R)p.yBottom;
+    R vx = R(0), vy = R(0), vz = R(0);
     if (move) {
-        const R hdt = (R)(0.5f * p.deltaT), dt = (R)p.deltaT;
-        R vx = vel[n], vy = vel[p.nMol + n], vz = vel[2 * p.nMol + n];
+        const R hdt = (R)(0.5f * p.deltaT);
+        vx = vel[n];
+        vy = vel[p.nMol + n];
+        vz = vel[2 * p.nMol + n];
         if (!kicked) {
             vx = vx + hdt * acc[n];
             vy = vy + hdt * acc[p.nMol + n];
@@ -42,17 +45,8 @@ __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename P
             vel[p.nMol + n] = vy;
             vel[2 * p.nMol + n] = vz;
         }
-        r.x = r.x + dt * vx;
-        r.y = r.y + dt * vy;
-        r.z = r.z + dt * vz;
     }
-    const R Lx = (R)p.Lx, Ly = (R)p.Ly, Lz = (R)p.Lz;
-    if (r.x >= R(0.5f) * Lx) r.x -= Lx;
-    else if (r.x < -R(0.5f) * Lx) r.x += Lx;
-    if (r.y >= R(0.5f) * Ly) r.y -= Ly;
-    else if (r.y < -R(0.5f) * Ly) r.y += Ly;
-    if (r.z >= R(0.5f) * Lz) r.z -= Lz;
-    else if (r.z < -R(0.5f) * Lz) r.z += Lz;
+    drift_wrap<R>(p, r, move, vx, vy, vz);
     pos[n] = r;
     if (doBin && (rebuild < 0 ? st->nebrNow : rebuild))
         bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
