@@ -36,6 +36,7 @@ struct BatchArgs {
     const int2 *nlTab;     // (replica, block) over each replica's cell-ordered list-build blocks
     const int *order;      // list-build order: replica r's atoms (local indices) at [atomOff, atomOff + nMol)
     void *pos, *vel, *acc;
+    void *posOut;          // the force pass's fused drift writes here (null: no drift)
     float *rho, *embF, *embDF, *fric;
     int *cellOf[2], *cnt[2], *slots[2], *slotOf;
     float4 *cellPos;

@@ -95,21 +95,37 @@ std::string batch_flags(const sgk_batch *b, int step0, int nsteps, int rdfEvery)
 void enqueue_batch(sgk_batch *b, const std::string &flags) {
     sgk::launch_batch_reset_records(b);
     // as enqueue_steps: the force pass applies the next step's first half-kick when no thermostat or
-    // shear comes in between and the step is not the chunk's last
-    bool kicked = false;
-    for (size_t s = 0; s < flags.size(); ++s) {
+    // shear comes in between and the step is not the chunk's last, and it also drifts the positions
+    // into posAlt on an even number of those steps (the first stays unfused when the count is odd),
+    // the host swapping pos and posAlt after each, so the chunk ends on the buffer it started on
+    const size_t n = flags.size();
+    std::vector<char> pre(n, 0);
+    int nPre = 0;
+    for (size_t s = 0; s < n; ++s) {
+        pre[s] = !((flags[s] - 'a') & 3) && s + 1 < n;
+        nPre += pre[s];
+    }
+    bool unfuseFirst = (nPre & 1) != 0;
+    bool kicked = false, drifted = false;
+    for (size_t s = 0; s < n; ++s) {
         const int k = flags[s] - 'a';
-        const bool prekick = !(k & 3) && s + 1 < flags.size();
-        sgk::launch_batch_lf1(b, kicked);
+        bool fuse = false;
+        if (pre[s] && b->posAlt) {
+            fuse = !unfuseFirst;
+            unfuseFirst = false;
+        }
+        sgk::launch_batch_lf1(b, kicked, drifted);
         sgk::launch_batch_sortcells(b);
         sgk::launch_batch_nl(b);
         sgk::launch_batch_rho(b);
-        sgk::launch_batch_force(b, prekick);
+        sgk::launch_batch_force(b, pre[s], fuse ? b->posAlt : nullptr);
         sgk::launch_batch_finalize(b);
-        kicked = prekick;
+        kicked = pre[s];
+        drifted = fuse;
         if (k & 1) sgk::launch_batch_thermostat(b);
         if (k & 2) sgk::launch_batch_shear(b);
-        if (k & 4) sgk::launch_batch_rdf(b);
+        if (k & 4) sgk::launch_batch_rdf(b);  // this step's positions, still in pos
+        if (fuse) std::swap(b->pos, b->posAlt);
     }
 }
 
@@ -319,6 +335,7 @@ int sgk_batch_create(int nrep, const SimParams *params, const sgk_config *cfgIn,
         BTRY(balloc(b, &b->ordBlock, (size_t)(ordOff + sgk::kOrdChunk - 1) / sgk::kOrdChunk, false));
     }
     BTRY(balloc(b, (char **)&b->pos, (size_t)atomOff * (b->mixed ? 32 : 16), true));
+    BTRY(balloc(b, (char **)&b->posAlt, (size_t)atomOff * (b->mixed ? 32 : 16), true));
     BTRY(balloc(b, (char **)&b->vel, (size_t)atomOff * 3 * Rb, true));
     BTRY(balloc(b, (char **)&b->acc, (size_t)atomOff * 3 * Rb, true));
     BTRY(balloc(b, &b->rho, (size_t)metalOff, true));

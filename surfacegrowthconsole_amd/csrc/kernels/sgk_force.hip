@@ -626,7 +626,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int2 e = a.forceTab[blockIdx.x];
     const BatchRep &B = a.reps[e.x];
     const DevParams &p = B.p;
-    force_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (typename Prec<M>::P4 *)a.pos + B.atomOff, nullptr,
+    force_block<M, T, G>(p, e.y, B.nbM, B.nbForce, false, (typename Prec<M>::P4 *)a.pos + B.atomOff,
+                         a.posOut ? (typename Prec<M>::P4 *)a.posOut + B.atomOff : nullptr,
                          (R *)a.vel + 3 * B.atomOff, (R *)a.acc + 3 * B.atomOff, a.nlM + B.metalOff * p.nlCap,
                          a.nnM + B.metalOff, a.embF + B.metalOff, a.embDF + B.metalOff, a.fric + B.metalOff,
                          a.lj + B.carbonOff * p.ljCap, a.nlj + B.carbonOff, a.bondC + B.carbonOff,
@@ -696,11 +697,12 @@ int batch_lane_index(int G) { return G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : G ==
 // One launch per lane count present: the replicas with G lanes per metal atom, through their own
 // block table (the kernel body is templated on G).
 template <bool M, class T>
-static void batch_dispatch(sgk_batch *b, bool force, int integrate) {
+static void batch_dispatch(sgk_batch *b, bool force, int integrate, void *posOut) {
     static const int kinds[sgk_batch::kLaneKinds] = {1, 2, 4, 8, 16};
     for (int q = 0; q < sgk_batch::kLaneKinds; ++q) {
         if (b->grpN[q] == 0) continue;
         BatchArgs a = b->args();
+        a.posOut = posOut;
         a.forceTab = b->forceTab + b->grpOff[q];
         const dim3 g(b->grpN[q]), t(256);
 #define SGK_B(GG)                                                                            \
@@ -721,18 +723,19 @@ static void batch_dispatch(sgk_batch *b, bool force, int integrate) {
     }
 }
 
-static void batch_launch(sgk_batch *b, bool force, int integrate) {
+static void batch_launch(sgk_batch *b, bool force, int integrate, void *posOut = nullptr) {
     if (b->nForceBlocks == 0) return;
+    if (integrate != 2) posOut = nullptr;  // the drift follows the next step's half-kick only
     if (!b->mixed)
-        batch_dispatch<false, float>(b, force, integrate);
+        batch_dispatch<false, float>(b, force, integrate, posOut);
     else if (b->cfg.pair_fp64)
-        batch_dispatch<true, double>(b, force, integrate);
+        batch_dispatch<true, double>(b, force, integrate, posOut);
     else
-        batch_dispatch<true, float>(b, force, integrate);
+        batch_dispatch<true, float>(b, force, integrate, posOut);
 }
 
 void launch_batch_rho(sgk_batch *b) { batch_launch(b, false, 0); }
-void launch_batch_force(sgk_batch *b, bool prekick) { batch_launch(b, true, prekick ? 2 : 1); }
+void launch_batch_force(sgk_batch *b, bool prekick, void *posOut) { batch_launch(b, true, prekick ? 2 : 1, posOut); }
 void launch_batch_force_eval(sgk_batch *b) { batch_launch(b, true, 0); }
 
 void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick, void *posOut) {

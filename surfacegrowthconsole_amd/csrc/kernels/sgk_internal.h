@@ -128,6 +128,7 @@ struct sgk_batch {
     int orderAge = 0;
     long long nAtoms = 0, nMetal = 0, nCarbon = 0, nCells = 0;
     void *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    void *posAlt = nullptr;  // where a force pass with the fused drift writes the next positions
     float *rho = nullptr, *embF = nullptr, *embDF = nullptr, *fric = nullptr;
     int *cellOf[2] = {nullptr, nullptr}, *cnt[2] = {nullptr, nullptr}, *slots[2] = {nullptr, nullptr};
     int *slotOf = nullptr;
@@ -157,6 +158,7 @@ struct sgk_batch {
         a.nlTab = nlTab;
         a.order = orderValid ? order : nullptr;
         a.pos = pos;
+        a.posOut = nullptr;
         a.vel = vel;
         a.acc = acc;
         a.rho = rho;
@@ -213,7 +215,7 @@ RdfGrid rdf_grid(const DevParams &p, int S);  // complete-range RDF grid, cells 
 void launch_rho(sgk_ctx *c);
 void launch_batch_rho(sgk_batch *b);
 int batch_lane_index(int G);  // position of G in the lane kinds 1, 2, 4, 8, 16
-void launch_batch_force(sgk_batch *b, bool prekick = false);
+void launch_batch_force(sgk_batch *b, bool prekick = false, void *posOut = nullptr);
 void launch_batch_force_eval(sgk_batch *b);  // forces, energies and partials only: velocities untouched
 // prekick: the force pass also applies the next step's first half-kick (the caller guarantees that
 // nothing touches velocities or accelerations before that step's launch_lf1(kicked = true)).
@@ -222,7 +224,8 @@ void launch_force(sgk_ctx *c, bool integrate, bool finalize, bool prekick = fals
 // sgk_step.hip
 void launch_lf1(sgk_ctx *c, bool bin, bool kicked = false);
 void launch_insert(sgk_ctx *c);
-void launch_batch_lf1(sgk_batch *b, bool kicked = false);
+// drifted: the previous force pass drifted the positions (posOut), so only the binning is left.
+void launch_batch_lf1(sgk_batch *b, bool kicked = false, bool drifted = false);
 void launch_batch_finalize(sgk_batch *b);
 void launch_batch_thermostat(sgk_batch *b);
 void launch_batch_shear(sgk_batch *b);

@@ -17,17 +17,24 @@ namespace sgk {
 // when the device flag asks for a rebuild, with the cell binning of the new position
 // (BinAtomsIntoCellsK, SurfaceGrowth.cu:547-609; see bin_atom).
 // kicked: the previous force pass already applied this step's half-kick (integrate 2), so only the
-// drift is left.  rebuild: whether this step bins (< 0: the replica's own flag, st->nebrNow).
+// drift is left.  drifted: it also did the drift (into the buffer pos now points to), so only the
+// binning is left.  rebuild: whether this step bins (< 0: the replica's own flag, st->nebrNow).
 template <bool M>
 __device__ __forceinline__ void lf1_body(const DevParams &p, int blk, typename Prec<M>::P4 *__restrict__ pos,
                                          typename Prec<M>::R *__restrict__ vel,
                                          const typename Prec<M>::R *__restrict__ acc, const CellBufs &bufs,
-                                         DevState *st, int doBin, int kicked, int rebuild = -1) {
+                                         DevState *st, int doBin, int kicked, int rebuild = -1,
+                                         int drifted = 0) {
 #pragma clang fp contract(off)
     using R = typename Prec<M>::R;
     const int n = blk * blockDim.x + threadIdx.x;
     if (n >= p.nMol) return;
     auto r = pos[n];
+    if (drifted) {
+        if (doBin && (rebuild < 0 ? st->nebrNow : rebuild))
+            bin_atom(p, n, make_float3((float)r.x, (float)r.y, (float)r.z), bufs, st);
+        return;
+    }
     bool move = true;
     if (n >= p.nMolMe)
         move = r.x != (R)p.xLeft && r.x < (R)p.xRight && r.y != (R)p.yTop && r.y != (R)p.yBottom;
@@ -67,7 +74,7 @@ __global__ __launch_bounds__(256) void k_lf1(DevParams p, typename Prec<M>::P4 *
 // a flag while it is being stored sees the replica's own decision or the OR, so it gets the same OR.
 // The flag is read and stored with relaxed atomics, so that overlap is defined behaviour.
 template <bool M>
-__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN) {
+__global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN, int drifted) {
     using P4 = typename Prec<M>::P4;
     using R = typename Prec<M>::R;
     const int2 e = a.atomTab[blockIdx.x];
@@ -89,7 +96,7 @@ __global__ __launch_bounds__(256) void kb_lf1(BatchArgs a, int kicked, int syncN
             __hip_atomic_store(&st->nebrNow, rebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     lf1_body<M>(B.p, e.y, (P4 *)a.pos + B.atomOff, (R *)a.vel + 3 * B.atomOff, (const R *)a.acc + 3 * B.atomOff,
-                rep_cells(a, B), st, 1, kicked, rebuild);
+                rep_cells(a, B), st, 1, kicked, rebuild, drifted);
 }
 
 // InsertAtomsK (SurfaceGrowth.cu:488-514) with the schedule of .cu:2148-2168, for the step about to
@@ -512,12 +519,14 @@ void launch_diffuse_reset(sgk_ctx *c) {
 }
 
 // ---- batched launchers -----------------------------------------------------------------------
-void launch_batch_lf1(sgk_batch *b, bool kicked) {
+void launch_batch_lf1(sgk_batch *b, bool kicked, bool drifted) {
     const int syncN = b->cfg.sync_rebuilds && b->nrep > 1 ? b->nrep : 0;
     if (b->mixed)
-        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN);
+        hipLaunchKernelGGL(kb_lf1<true>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN,
+                           (int)drifted);
     else
-        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN);
+        hipLaunchKernelGGL(kb_lf1<false>, dim3(b->nAtomBlocks), dim3(256), 0, b->stream, b->args(), (int)kicked, syncN,
+                           (int)drifted);
 }
 
 // One workgroup per replica closes the step (the rebuild flags of synchronised rebuilds are ORed at

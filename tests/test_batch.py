@@ -87,6 +87,34 @@ def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_batch_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
+    """In 40-step chunks the batched force pass drifts the positions of every pre-kicked step but at
+    most one (kb_lf1 then only bins); in chunks of two steps nothing is fused and kb_lf1 drifts.
+    With synchronised rebuilds the two batches agree bit for bit: every replica's positions with
+    their energies, velocities, accelerations and records."""
+    systems = replicas()
+    out = []
+    for chunk in (40, 2):
+        with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0,
+                      sync_rebuilds=1) as bt:
+            for k, s in enumerate(systems):
+                bt.upload(k, s.r, s.v, s.a)
+                bt.set_run_state(k, 0, 1, 0.0)
+            recs = [[] for _ in systems]
+            for _ in range(80 // chunk):
+                for k, x in enumerate(bt.run(chunk)):
+                    recs[k] += x
+            out.append(([bt.download(k) for k in range(len(systems))], recs))
+    (d0, r0), (d1, r1) = out
+    assert any(x["rebuilt"] for x in r0[0])
+    for k in range(len(systems)):
+        for a, b in zip(d0[k], d1[k]):
+            assert np.array_equal(a, b), k
+        assert r0[k] == r1[k], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_batch_with_a_metal_free_and_a_one_atom_replica(gpu, prec):
     """A replica without metal (no metal workgroups in any pass) and one with a single metal atom,
     beside an ordinary one, with synchronised rebuilds: each equals the replica run alone with the
