@@ -89,12 +89,13 @@ def test_batch_equals_replicas_run_alone(gpu, prec, nl_scheme):
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_batch_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
     """In 40-step chunks the batched force pass drifts the positions of every pre-kicked step but at
-    most one (kb_lf1 then only bins); in chunks of two steps nothing is fused and kb_lf1 drifts.
-    With synchronised rebuilds the two batches agree bit for bit: every replica's positions with
-    their energies, velocities, accelerations and records."""
+    most one (kb_lf1 then only bins); in chunks of four steps the first pre-kicked step is unfused
+    and the next two fused; in chunks of two steps nothing is fused and kb_lf1 drifts.  With
+    synchronised rebuilds the batches agree bit for bit with the last: every replica's positions
+    with their energies, velocities, accelerations and records."""
     systems = replicas()
     out = []
-    for chunk in (40, 2):
+    for chunk in (40, 4, 2):
         with sg.Batch([s.params for s in systems], precision=prec, rebuild_every=10, metal_lanes=4, xcd_map=0,
                       sync_rebuilds=1) as bt:
             for k, s in enumerate(systems):
@@ -105,12 +106,13 @@ def test_batch_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
                 for k, x in enumerate(bt.run(chunk)):
                     recs[k] += x
             out.append(([bt.download(k) for k in range(len(systems))], recs))
-    (d0, r0), (d1, r1) = out
-    assert any(x["rebuilt"] for x in r0[0])
-    for k in range(len(systems)):
-        for a, b in zip(d0[k], d1[k]):
-            assert np.array_equal(a, b), k
-        assert r0[k] == r1[k], k
+    d1, r1 = out[-1]
+    assert any(x["rebuilt"] for x in r1[0])
+    for d0, r0 in out[:-1]:
+        for k in range(len(systems)):
+            for a, b in zip(d0[k], d1[k]):
+                assert np.array_equal(a, b), k
+            assert r0[k] == r1[k], k
 
 
 @pytest.mark.gpu

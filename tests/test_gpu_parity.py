@@ -450,13 +450,14 @@ def test_graph_and_eager_bitwise_identical(gpu):
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
     """In a chunk of 40 steps the force pass of every pre-kicked step but at most one also drifts
-    the positions into the second position buffer.  A chunk of two steps holds at most one
-    pre-kicked step, which is never fused, so k_lf1 drifts every step there.  The two runs agree
-    bit for bit: positions with their per-atom energies, velocities, accelerations and every
+    the positions into the second position buffer.  A chunk of five steps holds four pre-kicked
+    steps, all fused; a chunk of four holds three, the first unfused and the next two fused; a chunk
+    of two holds one, never fused, so k_lf1 drifts every step there.  All runs agree bit for bit
+    with the last: positions with their per-atom energies, velocities, accelerations and every
     record, through the list rebuilds of 80 steps."""
     s = system("small")
     out = []
-    for chunk in (40, 2):
+    for chunk in (40, 5, 4, 2):
         with sg.Engine(s.params, precision=prec) as eng:
             eng.upload(s.r, s.v, s.a)
             eng.set_run_state(0, 1, 0.0)
@@ -464,11 +465,12 @@ def test_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
             for _ in range(80 // chunk):
                 recs += eng.run(chunk)
             out.append((eng.download(), recs))
-    (d0, r0), (d1, r1) = out
-    assert any(x["rebuilt"] for x in r0)
-    for a, b in zip(d0, d1):
-        assert np.array_equal(a, b)
-    assert r0 == r1
+    d1, r1 = out[-1]
+    assert any(x["rebuilt"] for x in r1)
+    for d0, r0 in out[:-1]:
+        for a, b in zip(d0, d1):
+            assert np.array_equal(a, b)
+        assert r0 == r1
 
 
 def test_autotune_keeps_results(gpu):
