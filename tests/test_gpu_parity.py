@@ -454,23 +454,27 @@ def test_fused_drift_bitwise_equals_the_leapfrog_kernel(gpu, prec):
     steps, all fused; a chunk of four holds three, the first unfused and the next two fused; a chunk
     of two holds one, never fused, so k_lf1 drifts every step there.  All runs agree bit for bit
     with the last: positions with their per-atom energies, velocities, accelerations and every
-    record, through the list rebuilds of 80 steps."""
+    record, through the list rebuilds of 80 steps.  The RDF sampled every 10 steps (on fused steps
+    too, where it must read the step's positions, not the drifted ones) agrees bin for bin."""
     s = system("small")
     out = []
     for chunk in (40, 5, 4, 2):
         with sg.Engine(s.params, precision=prec) as eng:
             eng.upload(s.r, s.v, s.a)
             eng.set_run_state(0, 1, 0.0)
+            eng.rdf_clear()
             recs = []
             for _ in range(80 // chunk):
-                recs += eng.run(chunk)
-            out.append((eng.download(), recs))
-    d1, r1 = out[-1]
+                recs += eng.run(chunk, rdf_every=10)
+            out.append((eng.download(), recs, eng.rdf()))
+    d1, r1, h1 = out[-1]
     assert any(x["rebuilt"] for x in r1)
-    for d0, r0 in out[:-1]:
+    assert h1.sum() > 0
+    for d0, r0, h0 in out[:-1]:
         for a, b in zip(d0, d1):
             assert np.array_equal(a, b)
         assert r0 == r1
+        assert np.array_equal(h0, h1)
 
 
 def test_autotune_keeps_results(gpu):
